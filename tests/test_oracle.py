@@ -1,0 +1,471 @@
+"""Pins the CPU FP64 oracle against closed forms, brute force, invariants
+and published known-answer vectors.
+
+The reference (KJLankinen/cubble) ships no tests, golden vectors or
+fixtures for this path, so every check here comes from the mathematics of
+Durian's bubble model or from a textbook routine:
+
+* RNG: SplitMix64 and xoshiro256** published test vectors;
+* pair search: O(N^2) brute force, permutation / translation invariance;
+* pair velocity: f0 (1/|d| - 1/R) d for two bubbles;
+* overlap area: lens-circle radius computed from the *other* sphere;
+* liquid-mediated gas flux: two-bubble closed form;
+* predictor / corrector: AB2 / trapezoid on a free bubble;
+* dt controller: scripted error (tolerance) sequences;
+* volume fix: sum V(r) = phi * V_box afterwards;
+* compaction: survivors and their pair set versus an explicit filter.
+"""
+import math
+
+import numpy as np
+import pytest
+
+PI = 3.1415926535897932384626433832795028841971693993
+
+
+def consts(dim, L=20.0, wall=(0, 0, 0), tol=1e-4, f0=1.0, kappa=0.01, k=0.1, ain=1.0,
+           vol_mul=1.0, skin=0.3, min_rad=0.1, drag=0.1, cell_cap=1 << 20):
+    iv = [L, L, L if dim == 3 else 0.0]
+    return {
+        "dim": dim, "wall": list(wall), "lbb": [0.0, 0.0, 0.0], "tfr": iv, "interval": iv,
+        "wall_drag": drag, "skin": skin, "min_rad": min_rad, "f0": f0, "k_param": k,
+        "kappa": kappa, "avg_area_in": ain, "vol_mul": vol_mul, "tol": tol, "cell_cap": cell_cap,
+    }
+
+
+def make_state(pos, r, dim, v=None, vo=None):
+    n = len(r)
+    x = np.zeros((4, n))
+    pos = np.asarray(pos, float)
+    x[:dim] = pos.T[:dim]
+    x[3] = r
+    st = {
+        "x": x, "v": np.zeros((4, n)) if v is None else np.asarray(v, float),
+        "vo": np.zeros((4, n)) if vo is None else np.asarray(vo, float),
+        "saved": x.copy(), "path": np.zeros(n), "err": np.zeros(n),
+        "wc": np.zeros((3, n), np.int32), "idx": np.arange(n, dtype=np.int32),
+        "nbr_off": np.zeros(n + 1, np.int32), "nbr": np.zeros(0, np.int32),
+    }
+    return st
+
+
+def scalars(n, dt=1e-4, max_r=1.0):
+    return {"count": n, "cell_dim": [1, 1, 1], "dt": dt, "time_int": 0, "time_frac": 0.0,
+            "max_r_cell": max_r, "steps": 0, "attempts": 0, "searches": 0, "removals": 0,
+            "bubble_steps": 0.0, "s_a": 0.0, "s_ar": 0.0, "s_oa": 0.0, "s_oar": 0.0,
+            "s_v": 0.0, "max_err": 0.0, "max_r": 0.0, "max_exp": 0.0, "n_del": 0,
+            "accepted": 0, "need_search": 0, "num_pairs": 0}
+
+
+def foam(ora, pos, r, dim, **kw):
+    c = consts(dim, **{k: v for k, v in kw.items() if k not in ("dt", "v", "vo")})
+    st = make_state(pos, r, dim, kw.get("v"), kw.get("vo"))
+    f = ora.Foam.from_state(c, st, scalars(len(r), dt=kw.get("dt", 1e-4), max_r=max(r)))
+    f.search()
+    return f
+
+
+# --------------------------------------------------------------------------
+# RNG known answers
+# --------------------------------------------------------------------------
+def test_splitmix64_published_vectors(ora):
+    got = [int(v) for v in ora.splitmix64(0, 3)]
+    assert got == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_xoshiro256ss_published_vectors(ora):
+    got = [int(v) for v in ora.xoshiro256ss([1, 2, 3, 4], 4)]
+    assert got == [11520, 0, 1509978240, 1215971899390074240]
+
+
+# --------------------------------------------------------------------------
+# deterministic cube root
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("a", [1e-300, 3.7e-9, 0.5, 1.0, 2.0, 8.0, 27.0, 1234.5678, 6.02e23, 1e300])
+def test_cbrt_close_to_libm(ora, a):
+    got = ora.cbrt(a)
+    want = np.cbrt(a)
+    assert abs(got - want) <= 2 * np.spacing(want)
+
+
+def test_cbrt_exact_cubes(ora):
+    for k in range(1, 200):
+        assert ora.cbrt(float(k ** 3)) == float(k)
+
+
+# --------------------------------------------------------------------------
+# pair search
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,n,phi,seed", [(2, 500, 0.9, 1), (2, 3000, 0.8, 2), (3, 800, 0.84, 3),
+                                            (3, 4000, 0.95, 4)])
+def test_cell_search_equals_brute_force(ora, dim, n, phi, seed):
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=phi, seed=seed, radius_dist=dim - 2, r_std=0.15)
+    assert np.array_equal(f.pairs(), f.brute_pairs())
+    st, sc = f.get_state()
+    # full lists are symmetric and ascending
+    off, nbr = st["nbr_off"], st["nbr"]
+    for i in range(0, n, max(1, n // 50)):
+        lst = nbr[off[i]:off[i + 1]]
+        assert np.all(np.diff(lst) > 0)
+        for j in lst:
+            assert i in nbr[off[j]:off[j + 1]]
+
+
+def test_cell_order_is_stable_counting_sort(ora):
+    f = ora.Foam.synthetic(dim=3, count=2000, phi=0.84, seed=7, radius_dist=1, r_std=0.15)
+    st, sc = f.get_state()
+    c = f.consts()
+    cd = sc["cell_dim"]
+    ids = []
+    for i in range(sc["count"]):
+        q = [min(max(int(math.floor(cd[k] * (st["x"][k][i] - c["lbb"][k]) / c["interval"][k])), 0),
+                 cd[k] - 1) for k in range(3)]
+        ids.append(q[2] * cd[0] * cd[1] + q[1] * cd[0] + q[0])
+    ids = np.array(ids)
+    assert np.all(np.diff(ids) >= 0)
+    # within a cell the original indices ascend (first search from identity order)
+    for cell in np.unique(ids)[:200]:
+        sel = st["idx"][ids == cell]
+        assert np.all(np.diff(sel) > 0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pair_set_invariant_to_permutation_and_translation(ora, dim):
+    rng = np.random.default_rng(11)
+    n = 600
+    L = 30.0 if dim == 3 else 60.0
+    pos = rng.uniform(0, L, size=(n, 3))
+    r = rng.uniform(0.7, 1.3, size=n)
+
+    def pair_set(pos, r, ids):
+        f = foam(ora, pos, r, dim, L=L)
+        st, _ = f.get_state()
+        p = ora.half_pairs(st["nbr_off"], st["nbr"])
+        orig = st["idx"]
+        return {tuple(sorted((ids[orig[a]], ids[orig[b]]))) for a, b in p}
+
+    base = pair_set(pos, r, np.arange(n))
+    perm = rng.permutation(n)
+    permuted = pair_set(pos[perm], r[perm], perm)
+    assert base == permuted
+    shift = np.array([7.25, -3.5, 11.0])
+    moved = np.mod(pos + shift, L)
+    assert pair_set(moved, r, np.arange(n)) == base
+
+
+@pytest.mark.parametrize("dim,L", [(2, 2.0), (2, 4.0), (2, 6.0), (3, 2.0), (3, 4.0), (3, 5.0)])
+def test_small_box_cell_dims_below_three(ora, dim, L):
+    """cellDim < 3 maps several stencil offsets onto one cell; the search must
+    still produce each minimum-image pair exactly once."""
+    rng = np.random.default_rng(5)
+    n = 40 if dim == 3 else 25
+    pos = rng.uniform(0, L, size=(n, 3))
+    r = rng.uniform(0.4, 0.8, size=n)
+    f = foam(ora, pos, r, dim, L=L)
+    assert min(f.scalars()["cell_dim"][:dim]) < 3
+    assert np.array_equal(f.pairs(), f.brute_pairs())
+
+
+def test_walls_do_not_wrap(ora):
+    # two bubbles at opposite faces: neighbours through a periodic face only
+    pos = [[0.2, 5.0, 5.0], [9.9, 5.0, 5.0]]
+    r = [0.5, 0.5]
+    per = foam(ora, pos, r, 3, L=10.0)
+    wal = foam(ora, pos, r, 3, L=10.0, wall=(1, 0, 0))
+    assert len(per.pairs()) == 1
+    assert len(wal.pairs()) == 0
+
+
+# --------------------------------------------------------------------------
+# pair force and overlap area: two-bubble closed forms
+# --------------------------------------------------------------------------
+def lens_a_from_first(r1, r2, d):
+    """Squared radius of the circle where two spheres meet, from sphere 1."""
+    x1 = (d * d + r1 * r1 - r2 * r2) / (2 * d)
+    return r1 * r1 - x1 * x1
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("r1,r2,d", [(1.0, 1.0, 1.7), (0.8, 1.3, 1.9), (1.2, 0.6, 1.5),
+                                     (1.0, 0.3, 0.5), (0.4, 1.1, 0.6)])
+def test_two_bubble_velocity_and_overlap(ora, dim, r1, r2, d):
+    L = 20.0
+    pos = [[5.0, 5.0, 5.0], [5.0 + d * 0.6, 5.0 + d * 0.8, 5.0]]
+    f0 = 1.7
+    f = foam(ora, pos, [r1, r2], dim, L=L, f0=f0, dt=1e-6, tol=1e9)
+    f.attempt(gas=True, path=False)
+    o = f.attempt_out()
+    st, _ = f.get_state()
+    # identify which slot holds which bubble after the cell sort
+    k1 = int(np.where(st["idx"] == 0)[0][0])
+    k2 = 1 - k1
+    dvec = np.array([0.6 * d, 0.8 * d])
+    u = f0 * (1.0 / d - 1.0 / (r1 + r2)) * dvec
+    assert np.allclose(o["vp"][:2, k2], u, rtol=1e-13, atol=1e-15)
+    assert np.allclose(o["vp"][:2, k1], -u, rtol=1e-13, atol=1e-15)
+    if d * d < r1 * r1 or d * d < r2 * r2:
+        a = min(r1, r2) ** 2
+    else:
+        a = abs(lens_a_from_first(r1, r2, d))
+    oa = PI * a if dim == 3 else 2.0 * math.sqrt(a)
+    assert o["ov"][k1] == pytest.approx(oa, rel=1e-12)
+    assert o["ov"][k2] == pytest.approx(oa, rel=1e-12)
+
+
+def test_non_overlapping_listed_pair_has_no_force(ora):
+    # inside the skin but not overlapping: listed, no contribution
+    f = foam(ora, [[5, 5, 5], [7.1, 5, 5]], [1.0, 1.0], 3, dt=1e-6, tol=1e9)
+    assert len(f.pairs()) == 1
+    f.attempt()
+    o = f.attempt_out()
+    assert np.all(o["vp"][:3] == 0.0) and np.all(o["ov"] == 0.0)
+
+
+# --------------------------------------------------------------------------
+# gas exchange
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_liquid_mediated_flux_two_free_bubbles(ora, dim):
+    r1, r2 = 0.8, 1.4
+    kappa, K, ain = 0.05, 0.1, 9.0
+    f = foam(ora, [[3, 3, 3], [12, 12, 12]], [r1, r2], dim, kappa=kappa, k=K, ain=ain, dt=1e-9,
+             tol=1e9)
+    f.attempt()
+    o = f.attempt_out()
+    st, _ = f.get_state()
+    r = st["x"][3]
+
+    def area(x):
+        return 2 * PI * x if dim == 2 else 4 * PI * x * x
+
+    s_a = area(r1) + area(r2)
+    s_ar = area(r1) / r1 + area(r2) / r2
+    inv_rho = s_ar / s_a
+    for k in range(2):
+        want = K * kappa * ain * 2 / s_a * (inv_rho - 1.0 / r[k])
+        assert o["vp"][3][k] == pytest.approx(want, rel=1e-12)
+    # volume is conserved by the RHS
+    assert abs(area(r[0]) * o["vp"][3][0] + area(r[1]) * o["vp"][3][1]) < 1e-15
+
+
+def test_single_bubble_radius_constant(ora):
+    f = foam(ora, [[3, 3, 3]], [1.1], 3, dt=1e-3, tol=1e9)
+    f.attempt()
+    assert f.attempt_out()["vp"][3][0] == 0.0
+
+
+@pytest.mark.parametrize("dim,n", [(2, 2000), (3, 3000)])
+def test_gas_rhs_conserves_volume_and_force_antisymmetric(ora, dim, n):
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=0.9, seed=3, radius_dist=dim - 2, r_std=0.15)
+    for _ in range(3):
+        f.integrate()
+    st, sc = f.get_state()
+    dt = sc["dt"]
+    f.attempt()
+    o = f.attempt_out()
+    # radii the attempt used: the AB2 predictor from the committed state
+    rpred = st["x"][3] + 0.5 * dt * (3.0 * st["v"][3] - st["vo"][3])
+    area = 2 * PI * rpred if dim == 2 else 4 * PI * rpred ** 2
+    flux = area * o["vp"][3]
+    assert abs(flux.sum()) <= 1e-10 * np.abs(flux).sum()
+    vsum = o["vp"][:dim].sum(axis=1)
+    vabs = np.abs(o["vp"][:dim]).sum(axis=1)
+    assert np.all(np.abs(vsum) <= 1e-12 * vabs)
+
+
+# --------------------------------------------------------------------------
+# predictor / corrector and the dt controller
+# --------------------------------------------------------------------------
+def test_ab2_predictor_and_trapezoid_corrector_free_bubble(ora):
+    dt = 0.01
+    v = np.zeros((4, 1))
+    vo = np.zeros((4, 1))
+    v[:3, 0] = [0.3, -0.2, 0.1]
+    vo[:3, 0] = [0.1, 0.4, -0.5]
+    f = foam(ora, [[5, 5, 5]], [1.0], 3, v=v, vo=vo, dt=dt, tol=1e9)
+    f.attempt(gas=False, path=True)
+    o = f.attempt_out()
+    x0 = np.array([5.0, 5.0, 5.0])
+    pred = x0 + 0.5 * dt * (3 * v[:3, 0] - vo[:3, 0])
+    corr = x0 + 0.5 * dt * (v[:3, 0] + 0.0)
+    assert np.allclose(o["xc"][:3, 0], corr, rtol=0, atol=1e-15)
+    assert f.scalars()["max_err"] == pytest.approx(np.max(np.abs(pred - corr)), rel=1e-12)
+    assert o["path_new"][0] == pytest.approx(np.linalg.norm(corr - x0), rel=1e-12)
+
+
+def test_dt_controller_reject_and_grow(ora):
+    # rejection: any error exceeds a tiny tolerance -> dt *= 0.37, no commit
+    v = np.zeros((4, 1))
+    v[0, 0] = 1.0
+    f = foam(ora, [[5, 5, 5]], [1.0], 3, v=v, dt=1e-3, tol=1e-300)
+    assert f.attempt() == 0
+    assert f.scalars()["dt"] == pytest.approx(1e-3 * 0.37, rel=1e-15)
+    assert f.attempt() == 0
+    assert f.scalars()["dt"] == pytest.approx(1e-3 * 0.37 * 0.37, rel=1e-15)
+    # growth: error < 0.45 tol and dt < 10 -> dt *= 1.269 for the next step; time uses it
+    g = foam(ora, [[5, 5, 5]], [1.0], 3, v=v, dt=1e-3, tol=1e30)
+    dts = []
+    t = 0.0
+    dt = 1e-3
+    for _ in range(5):
+        g.integrate()
+        dt *= 1.269
+        t += dt
+        dts.append(g.scalars()["dt"])
+    assert np.allclose(dts, [1e-3 * 1.269 ** k for k in range(1, 6)], rtol=1e-14)
+    sc = g.scalars()
+    assert sc["time_int"] + sc["time_frac"] == pytest.approx(t, rel=1e-14)
+    # growth stops once dt >= 10
+    h = foam(ora, [[5, 5, 5]], [1.0], 3, v=v, dt=9.0, tol=1e30)
+    h.integrate()
+    assert h.scalars()["dt"] == pytest.approx(9.0 * 1.269)
+    h.integrate()
+    assert h.scalars()["dt"] == pytest.approx(9.0 * 1.269)
+
+
+def test_error_between_tolerances_keeps_dt(ora):
+    v = np.zeros((4, 1))
+    v[0, 0] = 1.0
+    # error = |AB2 - trapezoid| = dt * |v| = 1e-3 ; tolerance 1.5e-3 -> between 0.45 tol and tol
+    f = foam(ora, [[5, 5, 5]], [1.0], 3, v=v, dt=1e-3, tol=1.5e-3)
+    assert f.attempt() == 1
+    assert f.scalars()["dt"] == 1e-3
+
+
+# --------------------------------------------------------------------------
+# wrap, removal, volume fix
+# --------------------------------------------------------------------------
+def test_wrap_counters_preserve_unwrapped_position(ora):
+    L = 10.0
+    pos = [[-0.5, 3.0, 10.25], [4.0, 10.75, 2.0], [9.0, -1.25, 5.0]]
+    f = foam(ora, [[1, 1, 1]] * 3, [0.5] * 3, 3, L=L)
+    st, sc = f.get_state()
+    st["x"][:3] = np.asarray(pos).T
+    f.set_state(st, sc)
+    f.search()
+    st2, _ = f.get_state()
+    order = np.argsort(st2["idx"])
+    x = st2["x"][:3, order]
+    wc = st2["wc"][:, order]
+    assert np.all((x >= 0) & (x <= L))
+    # x += L * m and wc -= m, so the unwrapped position is x + wc * L
+    assert np.allclose(x + wc * L, np.asarray(pos).T, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,n,frac", [(2, 1500, 0.02), (3, 2500, 0.05), (3, 2000, 0.3)])
+def test_removal_compaction_and_volume_fix(ora, dim, n, frac):
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=0.9, seed=9, radius_dist=dim - 2, r_std=0.15)
+    st, sc = f.get_state()
+    rng = np.random.default_rng(1)
+    kill = rng.random(n) < frac
+    st["x"][3][kill] = 0.05  # below minRad = 0.1: marked by the corrector
+    f.set_state(st, sc)
+    pairs_before = {tuple(p) for p in f.pairs()}
+    idx_before = st["idx"].copy()
+    # one attempt marks the bubbles below minRad; remove() then compacts the
+    # (uncommitted) current state with those marks and S_V
+    f.attempt()
+    o = f.attempt_out()
+    dele = o["del"].astype(bool)
+    assert dele.sum() >= kill.sum()
+    assert np.all(dele[kill])
+    sv = f.scalars()["s_v"]
+    nd = int(dele.sum())
+    st_pre, _ = f.get_state()
+    f.remove()
+    st2, sc2 = f.get_state()
+    nnew = n - nd
+    assert sc2["count"] == nnew
+    survivors = idx_before[~dele]
+    assert sorted(st2["idx"].tolist()) == sorted(survivors.tolist())
+    # the first nnew slots keep survivors in place; holes get tail survivors, last first
+    holes = [i for i in np.where(dele)[0] if i < nnew]
+    tail = [k for k in range(n - 1, nnew - 1, -1) if not dele[k]]
+    expect_idx = idx_before[:nnew].copy()
+    for h, s in zip(holes, tail):
+        expect_idx[h] = idx_before[s]
+    assert np.array_equal(st2["idx"], expect_idx)
+    # pair set: old pairs among survivors, expressed in original ids
+    pairs_after = {tuple(sorted((st2["idx"][a], st2["idx"][b]))) for a, b in
+                   ora.half_pairs(st2["nbr_off"], st2["nbr"])}
+    want = {tuple(sorted((idx_before[a], idx_before[b]))) for a, b in pairs_before
+            if not dele[a] and not dele[b]}
+    assert pairs_after == want
+    # volume fix scaling law r_new = r_old * M / S_V^(1/D) (the full
+    # phi * V_box restoration is checked through integrate() below)
+    c = f.consts()
+    vm = c["vol_mul"] / (sv ** (1.0 / dim))
+    order_old = {int(i): k for k, i in enumerate(idx_before)}
+    r_old = np.array([st_pre["x"][3][order_old[int(i)]] for i in st2["idx"]])
+    assert np.allclose(st2["x"][3], r_old * vm, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_volume_fix_restores_phi_in_integrate(ora, dim):
+    n = 2000
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=0.85, seed=21, radius_dist=dim - 2, r_std=0.15)
+    st, sc = f.get_state()
+    st["x"][3][::97] = 0.02
+    f.set_state(st, sc)
+    f.integrate()
+    sc2 = f.scalars()
+    assert sc2["removals"] == 1 and sc2["count"] < n
+    st2, _ = f.get_state()
+    c = f.consts()
+    vbox = np.prod(c["interval"][:dim])
+    r = st2["x"][3]
+    vol = (PI * r ** 2).sum() if dim == 2 else (4.0 / 3.0 * PI * r ** 3).sum()
+    assert abs(vol - 0.85 * vbox) / (0.85 * vbox) < 1e-10
+
+
+# --------------------------------------------------------------------------
+# synthetic initial state
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,dist,phi", [(2, 0, 0.9), (3, 1, 0.84)])
+def test_synthetic_init_box_and_distributions(ora, dim, dist, phi):
+    n = 20000
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=phi, seed=426, radius_dist=dist,
+                           r_std=0.1 if dist == 0 else 0.15)
+    st, sc = f.get_state()
+    c = f.consts()
+    r = st["x"][3]
+    vbox = np.prod(c["interval"][:dim])
+    vol = (PI * r ** 2).sum() if dim == 2 else (4.0 / 3.0 * PI * r ** 3).sum()
+    assert vol / vbox == pytest.approx(phi, rel=1e-12)
+    assert np.all((st["x"][:dim] >= 0) & (st["x"][:dim] < np.array(c["interval"][:dim])[:, None]))
+    if dist == 0:
+        assert r.min() >= 0.5 and r.max() <= 1.5
+        assert abs(r.mean() - 1.0) < 0.005 and abs(r.std() - 0.1) < 0.005
+    else:
+        assert abs(np.median(np.log(r))) < 0.005 and abs(np.log(r).std() - 0.15) < 0.005
+    area = 2 * PI * r if dim == 2 else 4 * PI * r ** 2
+    assert c["avg_area_in"] == pytest.approx(area.mean(), rel=1e-12)
+    m = math.sqrt(phi * vbox / PI) if dim == 2 else (0.75 * phi * vbox / PI) ** (1 / 3)
+    assert c["vol_mul"] == pytest.approx(m, rel=1e-14)
+    assert c["skin"] == 0.3 and c["min_rad"] == pytest.approx(0.1) and c["f0"] == 1.0
+
+
+def test_integrate_runs_and_counts(ora):
+    f = ora.Foam.synthetic(dim=2, count=1000, seed=426)
+    for _ in range(30):
+        f.integrate()
+    sc = f.scalars()
+    assert sc["steps"] == 30 and sc["attempts"] >= 30
+    assert sc["bubble_steps"] == 30 * 1000
+    st, _ = f.get_state()
+    assert np.all(np.isfinite(st["x"])) and np.all(np.isfinite(st["v"]))
+
+
+def test_openmp_matches_serial_closely(ora):
+    a = ora.Foam.synthetic(dim=3, count=4000, seed=5, radius_dist=1, r_std=0.15, phi=0.84)
+    b = ora.Foam.synthetic(dim=3, count=4000, seed=5, radius_dist=1, r_std=0.15, phi=0.84)
+    a.run(5)
+    ora.set_threads(4)
+    try:
+        b.run(5)
+    finally:
+        ora.set_threads(1)
+    sa, _ = a.get_state()
+    sb, _ = b.get_state()
+    assert np.array_equal(sa["idx"], sb["idx"])
+    assert np.allclose(sa["x"], sb["x"], rtol=1e-12, atol=1e-12)
