@@ -1,0 +1,442 @@
+"""cubble_amd: MI355X-native per-timestep path of the cubble wet-foam simulator.
+
+Python mirror of the reference's simulator step interface (KJLankinen/cubble,
+src/simulator.cu:60-337) over the C ABI in include/cubble_amd.h.  Everything
+computational runs in the HIP library ``libcubble_amd.so`` built in-tree by
+``python -m cubble_amd.build``; there is no CPU fallback -- if the library is
+missing, every call raises.
+
+    sim = Simulator(Config.workload("foam3d_1e6"))
+    res = sim.integrate(IntegrationParams(use_gas_exchange=True, increment_path=True))
+    sim.run_steps(1000)
+    state, scalars = sim.get_state()
+"""
+import ctypes
+import dataclasses
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcubble_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "cubble_amd.h")
+
+OK = 0
+E_INVALID, E_HIP, E_NOMEM, E_IO, E_UNSUPPORTED, E_CAPACITY, E_STATE = -1, -2, -3, -4, -5, -6, -7
+PREC_FP64, PREC_MIXED = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class CubbleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cubble_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _IntegrateParams(ctypes.Structure):
+    _fields_ = [("use_gas_exchange", ctypes.c_int32), ("use_flow", ctypes.c_int32),
+                ("increment_path", ctypes.c_int32)]
+
+
+class _StepResult(ctypes.Structure):
+    _fields_ = [
+        ("error_too_large", ctypes.c_int32), ("num_to_be_deleted", ctypes.c_int32),
+        ("count", ctypes.c_int32), ("num_pairs", ctypes.c_int32),
+        ("need_search", ctypes.c_int32), ("halted", ctypes.c_int32),
+        ("max_error", ctypes.c_double), ("max_radius", ctypes.c_double),
+        ("max_expansion", ctypes.c_double), ("dt", ctypes.c_double), ("time", ctypes.c_double),
+        ("steps", ctypes.c_int64), ("attempts", ctypes.c_int64), ("searches", ctypes.c_int64),
+        ("removals", ctypes.c_int64), ("bubble_steps", ctypes.c_double),
+    ]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("num_start", ctypes.c_int32), ("num_end", ctypes.c_int32),
+        ("radius_dist", ctypes.c_int32), ("r_mean", ctypes.c_double), ("r_std", ctypes.c_double),
+        ("r_lo", ctypes.c_double), ("r_hi", ctypes.c_double), ("rel_dim", ctypes.c_double * 3),
+        ("wall", ctypes.c_int32 * 3), ("wall_drag", ctypes.c_double), ("phi", ctypes.c_double),
+        ("mu", ctypes.c_double), ("sigma", ctypes.c_double), ("kappa", ctypes.c_double),
+        ("k_param", ctypes.c_double), ("error_tolerance", ctypes.c_double),
+        ("dt0", ctypes.c_double), ("seed", ctypes.c_uint64), ("precision", ctypes.c_int32),
+        ("nbr_cap", ctypes.c_int32), ("cell_cap_factor", ctypes.c_double),
+        ("max_steps", ctypes.c_int64),
+    ]
+
+
+class _StateView(ctypes.Structure):
+    _fields_ = [("x", _dp * 4), ("v", _dp * 4), ("vo", _dp * 4), ("saved", _dp * 4),
+                ("path", _dp), ("error", _dp), ("wrap", _ip * 3), ("index", _ip),
+                ("nbr_off", _ip), ("nbr", _ip), ("nbr_cap", ctypes.c_int64)]
+
+
+class _AttemptView(ctypes.Structure):
+    _fields_ = [("xc", _dp * 4), ("vp", _dp * 4), ("overlap", _dp), ("path_new", _dp),
+                ("deleted", _u8p)]
+
+
+class _Scalars(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("count", ctypes.c_int32), ("capacity", ctypes.c_int32),
+        ("nbr_cap", ctypes.c_int32), ("cell_cap", ctypes.c_int32),
+        ("cell_dim", ctypes.c_int32 * 3), ("wall", ctypes.c_int32 * 3),
+        ("precision", ctypes.c_int32),
+        ("lbb", ctypes.c_double * 3), ("tfr", ctypes.c_double * 3),
+        ("interval", ctypes.c_double * 3),
+        ("wall_drag", ctypes.c_double), ("skin", ctypes.c_double), ("min_rad", ctypes.c_double),
+        ("f0", ctypes.c_double), ("k_param", ctypes.c_double), ("kappa", ctypes.c_double),
+        ("avg_area_in", ctypes.c_double), ("vol_mul", ctypes.c_double), ("tol", ctypes.c_double),
+        ("dt", ctypes.c_double), ("time_int", ctypes.c_uint64), ("time_frac", ctypes.c_double),
+        ("max_r_cell", ctypes.c_double), ("steps", ctypes.c_int64),
+        ("attempts", ctypes.c_int64), ("searches", ctypes.c_int64),
+        ("removals", ctypes.c_int64), ("bubble_steps", ctypes.c_double),
+        ("s_a", ctypes.c_double), ("s_ar", ctypes.c_double), ("s_oa", ctypes.c_double),
+        ("s_oar", ctypes.c_double), ("s_v", ctypes.c_double), ("max_err", ctypes.c_double),
+        ("max_r", ctypes.c_double), ("max_exp", ctypes.c_double), ("n_del", ctypes.c_int32),
+        ("accepted", ctypes.c_int32), ("need_search", ctypes.c_int32),
+        ("num_pairs", ctypes.c_int32),
+    ]
+
+
+class _KernelTimes(ctypes.Structure):
+    _fields_ = [("predict_ms", ctypes.c_double), ("pair_ms", ctypes.c_double),
+                ("correct_ms", ctypes.c_double), ("reps", ctypes.c_int32)]
+
+
+# exported symbols of include/cubble_amd.h (checked by the ABI test)
+EXPORTS = [
+    "cubble_create", "cubble_create_from_json", "cubble_config_from_json",
+    "cubble_config_default", "cubble_destroy", "cubble_last_error", "cubble_step",
+    "cubble_integrate", "cubble_run_steps", "cubble_search_neighbors", "cubble_remove_bubbles",
+    "cubble_get_scalars", "cubble_get_state", "cubble_get_attempt", "cubble_set_state",
+    "cubble_generate_foam", "cubble_run", "cubble_version", "cubble_time_kernels",
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcubble_amd.so is not built (python -m cubble_amd.build); "
+                               "there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        P = ctypes.POINTER
+        L.cubble_create.argtypes = [P(_Config), P(vp)]
+        L.cubble_create_from_json.argtypes = [ctypes.c_char_p, P(vp)]
+        L.cubble_config_from_json.argtypes = [ctypes.c_char_p, P(_Config)]
+        L.cubble_config_default.argtypes = [P(_Config)]
+        L.cubble_destroy.argtypes = [vp]
+        L.cubble_last_error.argtypes = [vp]
+        L.cubble_last_error.restype = ctypes.c_char_p
+        for name in ("cubble_step", "cubble_integrate"):
+            getattr(L, name).argtypes = [vp, P(_IntegrateParams), P(_StepResult)]
+        L.cubble_run_steps.argtypes = [vp, P(_IntegrateParams), ctypes.c_int64, P(_StepResult)]
+        L.cubble_search_neighbors.argtypes = [vp]
+        L.cubble_remove_bubbles.argtypes = [vp]
+        L.cubble_get_scalars.argtypes = [vp, P(_Scalars)]
+        L.cubble_get_state.argtypes = [vp, P(_StateView)]
+        L.cubble_get_attempt.argtypes = [vp, P(_AttemptView)]
+        L.cubble_set_state.argtypes = [vp, P(_StateView), P(_Scalars)]
+        L.cubble_generate_foam.argtypes = [P(_Config), _dp, P(_Scalars)]
+        L.cubble_run.argtypes = [ctypes.c_int, P(ctypes.c_char_p)]
+        L.cubble_version.restype = ctypes.c_char_p
+        L.cubble_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
+                                          P(_KernelTimes)]
+        _lib = L
+    return _lib
+
+
+def _check(rc, handle=None):
+    if rc != OK:
+        msg = lib().cubble_last_error(handle)
+        raise CubbleError(rc, msg.decode() if msg else "")
+
+
+@dataclasses.dataclass
+class IntegrationParams:
+    """Mirror of the reference's IntegrationParams flags (src/data_definitions.h:227-236)."""
+    use_gas_exchange: bool = True
+    use_flow: bool = False
+    increment_path: bool = True
+
+    def _c(self):
+        return _IntegrateParams(int(self.use_gas_exchange), int(self.use_flow),
+                                int(self.increment_path))
+
+
+@dataclasses.dataclass
+class Config:
+    """Synthetic initial foam + constants; field meanings follow input_parameters.json."""
+    dim: int = 2
+    num_start: int = 100000
+    num_end: int = 1000
+    radius_dist: int = 0          # 0 clipped normal, 1 log-normal
+    r_mean: float = 1.0
+    r_std: float = 0.21
+    r_lo: float = 0.5
+    r_hi: float = 1.5
+    rel_dim: tuple = (1.0, 1.0, 1.0)
+    wall: tuple = (0, 0, 0)
+    wall_drag: float = 0.1
+    phi: float = 0.9
+    mu: float = 1.0
+    sigma: float = 1.0
+    kappa: float = 0.1
+    k_param: float = 0.001
+    error_tolerance: float = 1e-5
+    dt0: float = 1e-4
+    seed: int = 426
+    precision: int = PREC_FP64
+    nbr_cap: int = 0
+    cell_cap_factor: float = 2.0
+    max_steps: int = 0
+
+    def _c(self):
+        c = _Config()
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if f.name in ("rel_dim", "wall"):
+                arr = getattr(c, f.name)
+                for k in range(3):
+                    arr[k] = v[k]
+            else:
+                setattr(c, f.name, v)
+        return c
+
+    @classmethod
+    def _from_c(cls, c):
+        kw = {}
+        for f in dataclasses.fields(cls):
+            v = getattr(c, f.name)
+            kw[f.name] = tuple(v) if f.name in ("rel_dim", "wall") else v
+        return cls(**kw)
+
+    @classmethod
+    def from_json(cls, path):
+        c = _Config()
+        _check(lib().cubble_config_from_json(path.encode(), ctypes.byref(c)))
+        return cls._from_c(c)
+
+    @classmethod
+    def workload(cls, name, **over):
+        """The named synthetic workloads of the benchmark configurations."""
+        base = dict(mu=1.0, sigma=1.0, kappa=0.01, k_param=0.1, error_tolerance=1e-4, dt0=1e-4,
+                    r_mean=1.0, seed=426)
+        if name == "foam2d_1e4":       # 2D, normal(1, 0.1) clipped [0.5, 1.5], phi 0.9
+            base.update(dim=2, num_start=10000, radius_dist=0, r_std=0.1, r_lo=0.5, r_hi=1.5,
+                        phi=0.9)
+        elif name == "foam3d_1e6":     # 3D, log-normal sigma 0.15, phi 0.84
+            base.update(dim=3, num_start=1000000, radius_dist=1, r_std=0.15, phi=0.84)
+        elif name == "foam3d_1e7":     # 3D, log-normal sigma 0.15, phi 0.9
+            base.update(dim=3, num_start=10000000, radius_dist=1, r_std=0.15, phi=0.9)
+        else:
+            raise ValueError("unknown workload %r" % name)
+        base.update(over)
+        return cls(**base)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _res(r):
+    return {name: getattr(r, name) for name, _ in _StepResult._fields_}
+
+
+def _scal(s):
+    out = {}
+    for name, _ in _Scalars._fields_:
+        v = getattr(s, name)
+        out[name] = list(v) if hasattr(v, "__len__") else v
+    return out
+
+
+class Simulator:
+    """One foam on the GPU.  Mirrors the reference's step interface:
+    step() (one attempt), integrate(), search_neighbors(), remove_bubbles()."""
+
+    def __init__(self, config=None, json_path=None):
+        h = ctypes.c_void_p()
+        if json_path is not None:
+            _check(lib().cubble_create_from_json(json_path.encode(), ctypes.byref(h)))
+        else:
+            cfg = config if config is not None else Config()
+            self.config = cfg
+            _check(lib().cubble_create(ctypes.byref(cfg._c()), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().cubble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _call(self, rc):
+        _check(rc, self._h)
+
+    def step(self, ip=None):
+        r = _StepResult()
+        self._call(lib().cubble_step(self._h, ctypes.byref((ip or IntegrationParams())._c()),
+                                     ctypes.byref(r)))
+        return _res(r)
+
+    def integrate(self, ip=None):
+        r = _StepResult()
+        self._call(lib().cubble_integrate(self._h, ctypes.byref((ip or IntegrationParams())._c()),
+                                          ctypes.byref(r)))
+        return _res(r)
+
+    def run_steps(self, n, ip=None):
+        r = _StepResult()
+        self._call(lib().cubble_run_steps(self._h, ctypes.byref((ip or IntegrationParams())._c()),
+                                          int(n), ctypes.byref(r)))
+        return _res(r)
+
+    def time_kernels(self, reps=20, ip=None):
+        """Average device ms of predictor / pair / corrector, from HIP events."""
+        t = _KernelTimes()
+        self._call(lib().cubble_time_kernels(self._h, ctypes.byref((ip or IntegrationParams())._c()),
+                                             int(reps), ctypes.byref(t)))
+        return {"predict_ms": t.predict_ms, "pair_ms": t.pair_ms, "correct_ms": t.correct_ms,
+                "reps": t.reps}
+
+    def search_neighbors(self):
+        self._call(lib().cubble_search_neighbors(self._h))
+
+    def remove_bubbles(self):
+        self._call(lib().cubble_remove_bubbles(self._h))
+
+    def scalars(self):
+        s = _Scalars()
+        self._call(lib().cubble_get_scalars(self._h, ctypes.byref(s)))
+        return _scal(s)
+
+    def get_state(self):
+        sc = self.scalars()
+        n = sc["count"]
+        st = {"x": np.zeros((4, n)), "v": np.zeros((4, n)), "vo": np.zeros((4, n)),
+              "saved": np.zeros((4, n)), "path": np.zeros(n), "err": np.zeros(n),
+              "wc": np.zeros((3, n), np.int32), "idx": np.zeros(n, np.int32),
+              "nbr_off": np.zeros(n + 1, np.int32)}
+        nbr = np.zeros(max(1, sc["nbr_cap"] * n), np.int32)
+        v = _StateView()
+        for k in range(4):
+            v.x[k], v.v[k] = _dptr(st["x"][k]), _dptr(st["v"][k])
+            v.vo[k], v.saved[k] = _dptr(st["vo"][k]), _dptr(st["saved"][k])
+        for k in range(3):
+            v.wrap[k] = _iptr(st["wc"][k])
+        v.path, v.error, v.index = _dptr(st["path"]), _dptr(st["err"]), _iptr(st["idx"])
+        v.nbr_off, v.nbr, v.nbr_cap = _iptr(st["nbr_off"]), _iptr(nbr), nbr.size
+        self._call(lib().cubble_get_state(self._h, ctypes.byref(v)))
+        st["nbr"] = nbr[: st["nbr_off"][n]].copy()
+        return st, sc
+
+    def get_attempt(self):
+        n = self.scalars()["count"]
+        o = {"xc": np.zeros((4, n)), "vp": np.zeros((4, n)), "ov": np.zeros(n),
+             "path_new": np.zeros(n), "del": np.zeros(n, np.uint8)}
+        a = _AttemptView()
+        for k in range(4):
+            a.xc[k], a.vp[k] = _dptr(o["xc"][k]), _dptr(o["vp"][k])
+        a.overlap, a.path_new = _dptr(o["ov"]), _dptr(o["path_new"])
+        a.deleted = o["del"].ctypes.data_as(_u8p)
+        self._call(lib().cubble_get_attempt(self._h, ctypes.byref(a)))
+        return o
+
+    def set_state(self, st, scalars):
+        n = int(scalars["count"])
+        keep = []
+
+        def d(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a
+
+        def i32(a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            keep.append(a)
+            return a
+
+        v = _StateView()
+        x, vv, vo, sv = d(st["x"]), d(st["v"]), d(st["vo"]), d(st["saved"])
+        for k in range(4):
+            v.x[k], v.v[k], v.vo[k], v.saved[k] = _dptr(x[k]), _dptr(vv[k]), _dptr(vo[k]), _dptr(sv[k])
+        wc = i32(st["wc"])
+        for k in range(3):
+            v.wrap[k] = _iptr(wc[k])
+        path, err, idx = d(st["path"]), d(st["err"]), i32(st["idx"])
+        v.path, v.error, v.index = _dptr(path), _dptr(err), _iptr(idx)
+        off = i32(st["nbr_off"])
+        nbr = i32(st["nbr"] if len(st["nbr"]) else [0])
+        v.nbr_off, v.nbr, v.nbr_cap = _iptr(off), _iptr(nbr), len(st["nbr"])
+        s = _Scalars()
+        for name, _ in _Scalars._fields_:
+            if name in scalars:
+                val = scalars[name]
+                if hasattr(getattr(s, name), "__len__"):
+                    arr = getattr(s, name)
+                    for k in range(3):
+                        arr[k] = val[k]
+                else:
+                    setattr(s, name, val)
+        s.count = n
+        self._call(lib().cubble_set_state(self._h, ctypes.byref(v), ctypes.byref(s)))
+
+    def pairs(self):
+        """Half-list pair set (i < j), the reference's `pairs` (src/kernels.cu:703-711)."""
+        st, _ = self.get_state()
+        return half_pairs(st["nbr_off"], st["nbr"])
+
+
+def half_pairs(off, nbr):
+    off = np.asarray(off)
+    nbr = np.asarray(nbr)
+    n = len(off) - 1
+    i = np.repeat(np.arange(n, dtype=np.int32), np.diff(off))
+    m = nbr > i
+    p = np.stack([i[m], nbr[m]], axis=1)
+    order = np.lexsort((p[:, 1], p[:, 0]))
+    return p[order]
+
+
+def generate_foam(config):
+    """Host-only: the initial arrays create() uploads (x, y, z, r rows) and the derived
+    constants.  Needs no GPU."""
+    n = config.num_start
+    x4 = np.zeros(4 * n)
+    s = _Scalars()
+    _check(lib().cubble_generate_foam(ctypes.byref(config._c()), _dptr(x4), ctypes.byref(s)))
+    return x4.reshape(4, n), _scal(s)
+
+
+def version():
+    return lib().cubble_version().decode()
+
+
+def run_cli(argv):
+    """cubble_run(argc, argv) -- the drop-in command line entry."""
+    arr = (ctypes.c_char_p * len(argv))(*[a.encode() for a in argv])
+    return lib().cubble_run(len(argv), arr)

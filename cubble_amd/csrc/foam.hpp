@@ -1,0 +1,123 @@
+// foam.hpp -- device-resident foam header shared by the HIP kernels and the
+// host runtime.
+//
+// One `Foam` record per simulated foam lives in device memory.  It holds the
+// constants of the foam, the *role* pointers that say which physical buffer
+// currently plays x, x_predicted, dxdt, dxdt_old ... (the reference swaps
+// these on the host, src/simulator.cu:96-147 and :258-301; here the device
+// control code swaps them so no host round trip is needed), the scratch
+// arrays of the step, and the dynamic scalars (dt, time, count, flags).
+//
+// Every kernel is launched over a block table: entry b = (foam, local block),
+// so the same kernels drive one foam or a batch of foams of uneven size.
+#pragma once
+
+#include <stdint.h>
+
+namespace cubble_amd {
+
+constexpr int kBlock = 256;         // threads per block, bubble- and cell-space
+constexpr int kMaxBlocks = 4096;    // bubble-space blocks per foam (partials)
+constexpr int kNumXcd = 8;          // MI355X: 8 XCDs, workgroups round-robin
+constexpr int kPartD = 8;           // doubles per block partial record
+constexpr int kPartI = 4;           // ints per block partial record
+
+// reference src/util.cuh:63
+constexpr double kPi = 3.1415926535897932384626433832795028841971693993;
+
+struct Consts {
+    int32_t dim;
+    int32_t wall[3];
+    double lbb[3], tfr[3], interval[3];
+    double wall_drag, skin, min_rad, f0, k_param, kappa, avg_area_in, vol_mul, tol;
+};
+
+// Role pointers: which buffer currently plays each part of the state.
+struct Roles {
+    double *x[4];        // current x, y, z, r
+    double *xp[4];       // predicted, then corrected (attempt output)
+    double *v[4];        // current velocities
+    double *vp[4];       // predicted velocities (attempt output)
+    double *vo[4];       // old velocities
+    double *vs[4];       // spare (search reorder target)
+    double *saved[4];    // positions at the last search
+    double *path, *path_new;
+    double *err, *err_alt;
+    int32_t *wc[3], *wc_alt[3];
+    int32_t *idx, *idx_alt;
+};
+
+struct Scratch {
+    double *ov;          // per-bubble overlap area sum
+    double *invr;        // per-bubble 1 / r_predicted (mixed-precision path)
+    float *xf;           // float4 per bubble: predicted position relative to saved + r
+    uint8_t *del;        // deletion mark of the last attempt
+    int32_t *nn;         // neighbour count
+    int32_t *nbr;        // neighbour lists, column-major [nbr_cap][cap]
+    int32_t *cell_of;    // cell of each bubble (old order)
+    int32_t *newcell;    // cell of each bubble (new order)
+    int32_t *cell_count;
+    int32_t *cell_start; // num_cells + 1
+    int32_t *cell_cursor;
+    int32_t *members;    // unsorted cell segments
+    double *part;        // bubble-space block partials [kMaxBlocks][kPartD]
+    int32_t *part_i;     // bubble-space block partials [kMaxBlocks][kPartI]
+    int32_t *cpart;      // cell-space block partials (sums, then offsets)
+    int32_t *del_list;   // sorted deleted indices
+    int32_t *plan_src;   // source (tail survivor) of hole k
+    int32_t *moved_to;   // new slot of tail survivor (old index - new count)
+};
+
+struct Dyn {
+    int32_t count;
+    int32_t cell_dim[3];
+    int32_t num_cells;
+    int32_t accepted;
+    int32_t need_search;
+    int32_t need_remove;
+    int32_t halt;        // device loop stop (limit reached or capacity error)
+    int32_t overflow;    // neighbour list capacity exceeded
+    int32_t n_del;
+    int32_t n_holes;
+    int32_t num_pairs;
+    int32_t max_nn;
+    int32_t pad0;
+    double dt;
+    uint64_t time_int;
+    double time_frac;
+    double max_r_cell;
+    int64_t steps, attempts, searches, removals;
+    int64_t step_limit;  // halt once steps reaches this (0 = none)
+    double bubble_steps;
+    double s_a, s_ar, s_oa, s_oar, s_v, max_err, max_r, max_exp;
+    double vol_fix;      // multiplier of the pending volume fix
+    uint32_t ticket[8];  // last-block-done counters, one per kernel kind
+};
+
+struct Foam {
+    Consts c;
+    Roles r;
+    Scratch s;
+    Dyn d;
+    int32_t cap;         // bubble capacity
+    int32_t nbr_cap;     // neighbour slots per bubble
+    int32_t cell_cap;    // cell capacity
+    int32_t nblk;        // bubble-space blocks
+    int32_t ipt;         // bubbles per thread in bubble-space kernels
+    int32_t cblk;        // cell-space blocks
+    int32_t cipt;        // cells per thread in cell-space kernels
+    int32_t precision;
+};
+
+// ticket slots
+enum Ticket { kTkPair = 0, kTkCorr = 1, kTkRmEmit = 2, kTkRmApply = 3, kTkCells = 4,
+              kTkScan = 5, kTkReorder = 6, kTkSearch = 7 };
+
+// launch flags
+enum Flags : uint32_t {
+    kGas = 1u,
+    kPath = 2u,
+    kCommit = 4u,
+};
+
+}  // namespace cubble_amd

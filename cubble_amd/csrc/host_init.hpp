@@ -1,0 +1,32 @@
+// host_init.hpp -- host-side configuration, JSON schema reader and the
+// synthetic initial foam (project RNG).  No GPU calls.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "cubble_amd.h"
+#include "foam.hpp"
+
+namespace cubble_amd {
+
+struct InitOut {
+    Consts c;
+    double max_r;
+    int32_t cell_cap;
+};
+
+// Initial cell grid capacity rule shared with the search kernels.
+void host_cell_dims(const Consts &c, double max_r, long long cap, int cd[3]);
+
+// Radii and positions of a synthetic foam; x4 receives 4 rows of `count`
+// doubles (x, y, z, r).
+int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &out,
+                  std::string &err);
+
+// input_parameters.json (reference schema) -> config.
+int parse_config_json(const char *path, cubble_config &cfg, std::string &err);
+
+void config_default(cubble_config &cfg);
+
+}  // namespace cubble_amd

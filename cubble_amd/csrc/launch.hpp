@@ -1,0 +1,42 @@
+// launch.hpp -- host-side launch wrappers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "foam.hpp"
+
+namespace cubble_amd {
+
+// A launch covers one or more foams.  bt_bub[b] = (foam, local block) for
+// the bubble-space grid, bt_cell[b] likewise for the cell-space grid.
+struct Grid {
+    Foam *foams;
+    const int2 *bt_bub;
+    int nb_bub;
+    const int2 *bt_cell;
+    int nb_cell;
+    int nfoams;
+    hipStream_t stream;
+};
+
+void launch_predict(const Grid &g, uint32_t flags);
+void launch_pair(const Grid &g, uint32_t flags);
+void launch_pair_mixed(const Grid &g, uint32_t flags);
+void launch_correct(const Grid &g, uint32_t flags);
+
+// removeBubbles: emit the sorted deletion list and plan the moves, then move
+// the tail survivors into the holes, filter / remap the neighbour lists and
+// apply the volume fix.  Runs only for foams whose need_remove flag is set.
+void launch_remove(const Grid &g);
+
+// searchNeighbors: wrap + cell ids + counts, two-pass scan, binning,
+// rank-and-scatter reorder, neighbour lists.  Runs only for foams whose
+// need_search flag is set.
+void launch_search(const Grid &g);
+// Only the neighbour-list kernel (after the list capacity was raised).
+void launch_search_lists(const Grid &g);
+
+// Set / clear the phase flags of every foam in the launch.
+void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear_halt);
+
+}  // namespace cubble_amd

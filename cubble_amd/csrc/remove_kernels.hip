@@ -1,0 +1,167 @@
+// remove_kernels.hip -- removeBubbles (reference src/simulator.cu:180-190).
+//
+//   k_rm_emit  : blocks holding marked bubbles write them, ascending, into the
+//                deletion list at their prefix offset (block counts come from
+//                the corrector's partials); the last block plans the moves:
+//                the k-th hole below the new count receives the k-th surviving
+//                bubble counted from the back (reference swapDataCountPairs,
+//                src/kernels.cu:775-857), and computes the volume-fix
+//                multiplier M / S_V^(1/D) (src/kernels.cu:861-867)
+//   k_rm_apply : every slot below the new count pulls its state (a hole from
+//                its tail survivor), drops neighbours that were deleted, maps
+//                moved neighbours to their new slots, keeps the list
+//                ascending, and applies the volume fix (reference
+//                addVolumeFixPairs, src/kernels.cu:859-936)
+//
+// The reference appends the deletion list with an atomic counter, so its
+// hole / survivor pairing depends on thread timing; a sorted list makes the
+// pairing deterministic.  Its race-prone pair swap-from-the-back
+// (src/kernels.cu:873-909) is replaced by per-bubble list filtering.
+#include "dev_util.hpp"
+#include "launch.hpp"
+
+namespace cubble_amd {
+
+__device__ void plan_moves(Foam *F) {
+    const Scratch S = F->s;
+    const int n = F->d.count, nd = F->d.n_del, nnew = n - nd;
+    int below = 0;
+    for (int k = threadIdx.x; k < nd; k += kBlock) below += S.del_list[k] < nnew ? 1 : 0;
+    const int h = block_sum_i(below);
+    int got = 0;
+    for (int top = n - 1; got < h && top >= nnew; top -= kBlock) {
+        const int i = top - (int)threadIdx.x;
+        const int f = (i >= nnew && !S.del[i]) ? 1 : 0;
+        int tot = 0;
+        const int pre = block_excl_scan(f, tot);
+        if (f && got + pre < h) {
+            S.plan_src[got + pre] = i;
+            S.moved_to[i - nnew] = S.del_list[got + pre];
+        }
+        got += tot;
+    }
+    if (threadIdx.x == 0) {
+        const double sv = F->d.s_v;
+        double vm = 3 == F->c.dim ? 1.0 / det_cbrt(sv) : 1.0 / sqrt(sv);
+        vm *= F->c.vol_mul;
+        F->d.vol_fix = vm;
+        F->d.n_holes = h;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rm_emit(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_remove) return;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int cnt = S.part_i[cx.lb * kPartI + 0];
+    if (cnt > 0) {
+        int acc = 0;
+        for (int b = threadIdx.x; b < cx.lb; b += kBlock) acc += S.part_i[b * kPartI + 0];
+        int off = block_sum_i(acc);
+        const int base = cx.lb * kBlock * ipt;
+        for (int k = 0; k < ipt; ++k) {
+            const int i = base + k * kBlock + (int)threadIdx.x;
+            const int f = (i < n && S.del[i]) ? 1 : 0;
+            int tot = 0;
+            const int pre = block_excl_scan(f, tot);
+            if (f) S.del_list[off + pre] = i;
+            off += tot;
+        }
+    }
+    if (last_block(&F->d.ticket[kTkRmEmit], F->nblk)) {
+        plan_moves(F);
+        if (threadIdx.x == 0) F->d.ticket[kTkRmEmit] = 0;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, size_t cap, int base,
+                                           int ipt, int nnew, int h, double vm) {
+    int sum_nn = 0;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= nnew) break;
+        int lo = 0, hi = h;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (S.del_list[mid] < i) lo = mid + 1;
+            else hi = mid;
+        }
+        const int src = (lo < h && S.del_list[lo] == i) ? S.plan_src[lo] : i;
+        if (src != i) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (2 == q && 2 == D) continue;
+                R.x[q][i] = R.x[q][src];
+                R.v[q][i] = R.v[q][src];
+                R.vo[q][i] = R.vo[q][src];
+                R.saved[q][i] = R.saved[q][src];
+            }
+            R.path[i] = R.path[src];
+            R.err[i] = R.err[src];
+#pragma unroll
+            for (int q = 0; q < D; ++q) R.wc[q][i] = R.wc[q][src];
+            R.idx[i] = R.idx[src];
+        }
+        const int m = S.nn[src];
+        int out = 0;
+        for (int q = 0; q < m; ++q) {
+            const int j = S.nbr[(size_t)q * cap + src];
+            if (S.del[j]) continue;
+            const int jj = j >= nnew ? S.moved_to[j - nnew] : j;
+            // insertion keeps the list ascending
+            int pos = out;
+            while (pos > 0) {
+                const int prev = S.nbr[(size_t)(pos - 1) * cap + i];
+                if (prev <= jj) break;
+                S.nbr[(size_t)pos * cap + i] = prev;
+                --pos;
+            }
+            S.nbr[(size_t)pos * cap + i] = jj;
+            ++out;
+        }
+        S.nn[i] = out;
+        R.x[3][i] = R.x[3][i] * vm;
+        sum_nn += out;
+    }
+    return sum_nn;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rm_apply(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_remove) return;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int nnew = n - F->d.n_del, h = F->d.n_holes;
+    const double vm = F->d.vol_fix;
+    const size_t cap = (size_t)F->cap;
+    const int base = cx.lb * kBlock * ipt;
+    int sum_nn = 3 == F->c.dim ? apply_range<3>(R, S, cap, base, ipt, nnew, h, vm)
+                               : apply_range<2>(R, S, cap, base, ipt, nnew, h, vm);
+    sum_nn = block_sum_i(sum_nn);
+    if (threadIdx.x == 0) S.part_i[cx.lb * kPartI + 1] = sum_nn;
+    if (last_block(&F->d.ticket[kTkRmApply], F->nblk)) {
+        const int nb = F->nblk;
+        int s = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) s += S.part_i[b * kPartI + 1];
+        s = block_sum_i(s);
+        if (threadIdx.x == 0) {
+            F->d.count = nnew;
+            F->d.removals += 1;
+            F->d.num_pairs = s / 2;
+            F->d.need_remove = 0;
+            F->d.ticket[kTkRmApply] = 0;
+        }
+    }
+}
+
+void launch_remove(const Grid &g) {
+    hipLaunchKernelGGL(k_rm_emit, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_rm_apply, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+}
+
+}  // namespace cubble_amd

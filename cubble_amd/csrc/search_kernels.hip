@@ -1,0 +1,389 @@
+// search_kernels.hip -- searchNeighbors (reference src/simulator.cu:60-178).
+//
+//   k_s_cells   : wrap positions into the box on periodic axes, cell id of
+//                 every bubble, cell counts (reference wrapOverPeriodicBoundaries
+//                 src/kernels.cu:992-1011, cellByPosition :598-607)
+//   k_s_scan1   : per-block cell-count sums; the last block scans them
+//   k_s_scan2   : cell offsets (exclusive scan), cursors, counts reset
+//                 (reference: cub::DeviceScan::InclusiveSum, simulator.cu:90-91)
+//   k_s_bin     : bubbles into unsorted cell segments (reference indexByCell
+//                 src/kernels.cu:609-615)
+//   k_s_reorder : each bubble ranks itself among its cell's members by old
+//                 index (a stable counting sort, where the reference's
+//                 atomic order is arbitrary) and scatters its whole state to
+//                 its new slot; saved <- current; the last block swaps the
+//                 role pointers (reference reorganizeByIndex :721-770 and the
+//                 pointer rotation in simulator.cu:96-147)
+//   k_s_search  : full neighbour list of every bubble, ascending, over the
+//                 de-duplicated 3^D cell neighbourhood (reference
+//                 neighborSearch / comparePair src/kernels.cu:617-701 with
+//                 getNeighborCellIndex :1116-1189); the reference's half list
+//                 is the subset j > i.  Counts beyond the list capacity are
+//                 reported, never written out of bounds.
+#include "dev_util.hpp"
+#include "launch.hpp"
+
+namespace cubble_amd {
+
+template <int D>
+__device__ __forceinline__ void cells_range(const Consts &c, const Roles &R, const Scratch &S,
+                                            const int cd[3], int base, int ipt, int n) {
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            double xv = R.x[q][i];
+            if (!c.wall[q]) {
+                const int mult = xv < c.lbb[q] ? 1 : (xv > c.tfr[q] ? -1 : 0);
+                xv = xv + (c.tfr[q] - c.lbb[q]) * (double)mult;
+                R.x[q][i] = xv;
+                if (mult) R.wc[q][i] -= mult;
+            }
+            p[q] = xv;
+        }
+        int id[3] = {0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            const double f = floor(cd[q] * (p[q] - c.lbb[q]) / c.interval[q]);
+            id[q] = f < 0.0 ? 0 : (f >= (double)cd[q] ? cd[q] - 1 : (int)f);
+        }
+        const int cid = id[2] * cd[0] * cd[1] + id[1] * cd[0] + id[0];
+        S.cell_of[i] = cid;
+        atomicAdd(&S.cell_count[cid], 1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_s_cells(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_search) return;
+    const Consts c = F->c;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    int cd[3];
+    cell_dims(c, F->d.max_r_cell, F->cell_cap, cd);
+    const int base = cx.lb * kBlock * ipt;
+    if (3 == c.dim) cells_range<3>(c, R, S, cd, base, ipt, n);
+    else cells_range<2>(c, R, S, cd, base, ipt, n);
+    if (last_block(&F->d.ticket[kTkCells], F->nblk)) {
+        if (threadIdx.x == 0) {
+            F->d.cell_dim[0] = cd[0];
+            F->d.cell_dim[1] = cd[1];
+            F->d.cell_dim[2] = cd[2];
+            F->d.num_cells = cd[0] * cd[1] * cd[2];
+            F->d.ticket[kTkCells] = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_s_scan1(Foam *foams, const int2 *bt) {
+    const Ctx cx = cell_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_search) return;
+    const Scratch S = F->s;
+    const int ncell = F->d.num_cells, cipt = F->cipt;
+    const int base = cx.lb * kBlock * cipt;
+    int s = 0;
+    for (int k = 0; k < cipt; ++k) {
+        const int q = base + k * kBlock + (int)threadIdx.x;
+        if (q < ncell) s += S.cell_count[q];
+    }
+    s = block_sum_i(s);
+    if (threadIdx.x == 0) S.cpart[cx.lb] = s;
+    if (last_block(&F->d.ticket[kTkScan], F->cblk)) {
+        // exclusive scan of the block sums, each thread owning a contiguous run
+        const int nb = F->cblk;
+        const int per = (nb + kBlock - 1) / kBlock;
+        const int b0 = (int)threadIdx.x * per;
+        int loc = 0;
+        for (int b = b0; b < b0 + per && b < nb; ++b) loc += S.cpart[b];
+        int tot = 0;
+        int off = block_excl_scan(loc, tot);
+        for (int b = b0; b < b0 + per && b < nb; ++b) {
+            const int v = S.cpart[b];
+            S.cpart[b] = off;
+            off += v;
+        }
+        if (threadIdx.x == 0) {
+            S.cell_start[ncell] = tot;
+            F->d.ticket[kTkScan] = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_s_scan2(Foam *foams, const int2 *bt) {
+    const Ctx cx = cell_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_search) return;
+    const Scratch S = F->s;
+    const int ncell = F->d.num_cells, cipt = F->cipt;
+    const int base = cx.lb * kBlock * cipt;
+    int off = S.cpart[cx.lb];
+    for (int k = 0; k < cipt; ++k) {
+        const int q = base + k * kBlock + (int)threadIdx.x;
+        const int v = q < ncell ? S.cell_count[q] : 0;
+        int tot = 0;
+        const int pre = block_excl_scan(v, tot);
+        if (q < ncell) {
+            S.cell_start[q] = off + pre;
+            S.cell_cursor[q] = off + pre;
+            S.cell_count[q] = 0;
+        }
+        off += tot;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_s_bin(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_search) return;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int base = cx.lb * kBlock * ipt;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        const int slot = atomicAdd(&S.cell_cursor[S.cell_of[i]], 1);
+        S.members[slot] = i;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, int base, int ipt,
+                                              int n) {
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        const int c = S.cell_of[i];
+        const int s0 = S.cell_start[c], s1 = S.cell_start[c + 1];
+        int rank = 0;
+        for (int s = s0; s < s1; ++s) rank += S.members[s] < i ? 1 : 0;
+        const int np = s0 + rank;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (2 == q && 2 == D) continue;
+            const double xv = R.x[q][i];
+            R.xp[q][np] = xv;
+            R.saved[q][np] = xv;
+            R.vp[q][np] = R.v[q][i];
+            R.vs[q][np] = R.vo[q][i];
+        }
+        R.path_new[np] = R.path[i];
+        R.err_alt[np] = R.err[i];
+#pragma unroll
+        for (int q = 0; q < D; ++q) R.wc_alt[q][np] = R.wc[q][i];
+        R.idx_alt[np] = R.idx[i];
+        S.newcell[np] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_s_reorder(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_search) return;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int base = cx.lb * kBlock * ipt;
+    if (3 == F->c.dim) reorder_range<3>(R, S, base, ipt, n);
+    else reorder_range<2>(R, S, base, ipt, n);
+    if (last_block(&F->d.ticket[kTkReorder], F->nblk)) {
+        if (threadIdx.x == 0) {
+            Roles &W = F->r;
+            for (int q = 0; q < 4; ++q) {
+                double *t = W.x[q];
+                W.x[q] = W.xp[q];
+                W.xp[q] = t;
+                t = W.v[q];
+                W.v[q] = W.vp[q];
+                W.vp[q] = t;
+                t = W.vo[q];
+                W.vo[q] = W.vs[q];
+                W.vs[q] = t;
+            }
+            double *t = W.path;
+            W.path = W.path_new;
+            W.path_new = t;
+            t = W.err;
+            W.err = W.err_alt;
+            W.err_alt = t;
+            for (int q = 0; q < 3; ++q) {
+                int *u = W.wc[q];
+                W.wc[q] = W.wc_alt[q];
+                W.wc_alt[q] = u;
+            }
+            int *u = W.idx;
+            W.idx = W.idx_alt;
+            W.idx_alt = u;
+            F->d.ticket[kTkReorder] = 0;
+        }
+    }
+}
+
+// Sorted, de-duplicated neighbour coordinates of cell coordinate `x` along
+// one axis: periodic axes wrap, wall axes drop out-of-range cells.  With
+// fewer than 3 cells the wrapped offsets coincide and appear once.
+__device__ __forceinline__ int axis_cells(int x, int n, bool wall, int out[3]) {
+    int a = x - 1, b = x, c = x + 1;
+    int cnt = 0;
+    if (wall) {
+        if (a >= 0) out[cnt++] = a;
+        out[cnt++] = b;
+        if (c < n) out[cnt++] = c;
+        return cnt;
+    }
+    a = (a + n) % n;
+    c = c % n;
+    // sort three values
+    int v0 = a, v1 = b, v2 = c, t;
+    if (v0 > v1) { t = v0; v0 = v1; v1 = t; }
+    if (v1 > v2) { t = v1; v1 = v2; v2 = t; }
+    if (v0 > v1) { t = v0; v0 = v1; v1 = t; }
+    out[cnt++] = v0;
+    if (v1 != v0) out[cnt++] = v1;
+    if (v2 != v1) out[cnt++] = v2;
+    return cnt;
+}
+
+// Full neighbour list of every bubble in ascending index order.  The linear
+// cell id z*cx*cy + y*cx + x grows lexicographically in (z, y, x), so walking
+// the sorted per-axis cell lists in that nesting visits cells -- and, since
+// bubbles are stored in cell order, candidates -- in ascending order.
+// Consecutive x-cells of one row are one contiguous range of bubbles.
+template <int D>
+__device__ __forceinline__ void search_range(const Consts &c, const Roles &R, const Scratch &S,
+                                             const int cd[3], size_t cap, int nbr_cap, int base,
+                                             int ipt, int n, int &sum_nn, int &max_nn, int &ovf) {
+    const double *__restrict__ px = R.x[0];
+    const double *__restrict__ py = R.x[1];
+    const double *__restrict__ pz = R.x[2];
+    const double *__restrict__ pr = R.x[3];
+    const int *__restrict__ start = S.cell_start;
+    const double skin = c.skin;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        const int ci = S.newcell[i];
+        const int ccx = ci % cd[0];
+        const int ccy = (ci / cd[0]) % cd[1];
+        const int ccz = ci / (cd[0] * cd[1]);
+        int xs[3], ys[3], zs[3];
+        const int nx = axis_cells(ccx, cd[0], c.wall[0] != 0, xs);
+        const int ny = axis_cells(ccy, cd[1], c.wall[1] != 0, ys);
+        int nz = 1;
+        zs[0] = 0;
+        if (3 == D) nz = axis_cells(ccz, cd[2], c.wall[2] != 0, zs);
+        const double xi = px[i], yi = py[i], zi = D == 3 ? pz[i] : 0.0, ri = pr[i];
+        int cnt = 0;
+        for (int a = 0; a < nz; ++a) {
+            for (int b = 0; b < ny; ++b) {
+                const int row = zs[a] * cd[0] * cd[1] + ys[b] * cd[0];
+                int e = 0;
+                while (e < nx) {
+                    int f = e;
+                    while (f + 1 < nx && xs[f + 1] == xs[f] + 1) ++f;
+                    const int j0 = start[row + xs[e]], j1 = start[row + xs[f] + 1];
+                    for (int j = j0; j < j1; ++j) {
+                        if (j == i) continue;
+                        const double maxd = ri + pr[j] + skin;
+                        double d[3];
+                        wrapped_diff<D>(c, xi, yi, zi, px[j], py[j], D == 3 ? pz[j] : 0.0, d);
+                        if (sq_len(d) < maxd * maxd) {
+                            if (cnt < nbr_cap) S.nbr[(size_t)cnt * cap + i] = j;
+                            ++cnt;
+                        }
+                    }
+                    e = f + 1;
+                }
+            }
+        }
+        S.nn[i] = cnt < nbr_cap ? cnt : nbr_cap;
+        sum_nn += cnt;
+        max_nn = max(max_nn, cnt);
+        if (cnt > nbr_cap) ovf = 1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_search) return;
+    const Consts c = F->c;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
+    const int base = cx.lb * kBlock * ipt;
+    int sum_nn = 0, max_nn = 0, ovf = 0;
+    if (3 == c.dim)
+        search_range<3>(c, R, S, cd, (size_t)F->cap, F->nbr_cap, base, ipt, n, sum_nn, max_nn, ovf);
+    else
+        search_range<2>(c, R, S, cd, (size_t)F->cap, F->nbr_cap, base, ipt, n, sum_nn, max_nn, ovf);
+    sum_nn = block_sum_i(sum_nn);
+    max_nn = block_max_i(max_nn);
+    ovf = block_max_i(ovf);
+    if (threadIdx.x == 0) {
+        S.part_i[cx.lb * kPartI + 1] = sum_nn;
+        S.part_i[cx.lb * kPartI + 2] = max_nn;
+        S.part_i[cx.lb * kPartI + 3] = ovf;
+    }
+    if (last_block(&F->d.ticket[kTkSearch], F->nblk)) {
+        const int nb = F->nblk;
+        long long s = 0;
+        int mx = 0, of = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) {
+            s += S.part_i[b * kPartI + 1];
+            mx = max(mx, S.part_i[b * kPartI + 2]);
+            of = max(of, S.part_i[b * kPartI + 3]);
+        }
+        const int tot = block_sum_i((int)s);
+        mx = block_max_i(mx);
+        of = block_max_i(of);
+        if (threadIdx.x == 0) {
+            F->d.num_pairs = tot / 2;
+            F->d.max_nn = mx;
+            F->d.searches += 1;
+            if (of) {
+                F->d.overflow = 1;
+                F->d.halt = 1;
+            }
+            F->d.need_search = 0;
+            F->d.ticket[kTkSearch] = 0;
+        }
+    }
+}
+
+__global__ void k_set_flags(Foam *foams, int nf, int ns, int nr, int clear_halt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    if (ns >= 0) foams[f].d.need_search = ns;
+    if (nr >= 0) foams[f].d.need_remove = nr;
+    if (clear_halt) {
+        foams[f].d.halt = 0;
+        foams[f].d.overflow = 0;
+    }
+}
+
+void launch_search(const Grid &g) {
+    hipLaunchKernelGGL(k_s_cells, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_s_scan1, dim3(g.nb_cell), dim3(kBlock), 0, g.stream, g.foams, g.bt_cell);
+    hipLaunchKernelGGL(k_s_scan2, dim3(g.nb_cell), dim3(kBlock), 0, g.stream, g.foams, g.bt_cell);
+    hipLaunchKernelGGL(k_s_bin, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_s_reorder, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_s_search, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+}
+
+void launch_search_lists(const Grid &g) {
+    hipLaunchKernelGGL(k_s_search, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+}
+
+void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear_halt) {
+    const int nb = (g.nfoams + 63) / 64;
+    hipLaunchKernelGGL(k_set_flags, dim3(nb), dim3(64), 0, g.stream, g.foams, g.nfoams, need_search,
+                       need_remove, clear_halt);
+}
+
+}  // namespace cubble_amd

@@ -1,0 +1,442 @@
+// step_kernels.hip -- the three kernels of one step() attempt.
+//
+//   k_predict : AB2 predictor + block partials of S_A, S_Ar
+//               (reference preIntegrate, src/kernels.cu:40-170)
+//   k_pair    : pairwise Durian velocity, overlap area and gas flux, gathered
+//               per bubble over its full neighbour list; the last block
+//               reduces S_A, S_Ar, S_OA, S_OAr
+//               (reference pairwiseInteraction, src/kernels.cu:172-337)
+//   k_correct : walls, liquid-mediated gas exchange, trapezoid corrector,
+//               error / expansion / radius / volume reductions, deletion
+//               marks, path; the last block runs the dt controller and, when
+//               committing, rotates the role pointers
+//               (reference postIntegrate, src/kernels.cu:339-495, :546-593,
+//                and step()/integrate(), src/simulator.cu:192-314)
+//
+// The reference scatters every pair into both bubbles with FP64 atomics in
+// unspecified order.  Here each bubble gathers its own neighbours in
+// ascending index order, so results are deterministic, independent of the
+// launch shape, and need no atomics.  All reductions are fixed-order trees.
+#include "dev_util.hpp"
+#include "launch.hpp"
+
+namespace cubble_amd {
+
+// ---------------------------------------------------------------- predictor
+template <int D, bool GAS>
+__device__ __forceinline__ void predict_range(const Roles &R, int base, int ipt, int n, double h,
+                                              double &sa, double &sar) {
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+#pragma unroll
+        for (int q = 0; q < D; ++q) R.xp[q][i] = R.x[q][i] + h * (3.0 * R.v[q][i] - R.vo[q][i]);
+        if (GAS) {
+            const double rp = R.x[3][i] + h * (3.0 * R.v[3][i] - R.vo[3][i]);
+            R.xp[3][i] = rp;
+            double apr = 2.0 * kPi;
+            if (3 == D) apr *= 2.0 * rp;
+            sa += apr * rp;
+            sar += apr;
+        } else {
+            R.xp[3][i] = R.x[3][i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_predict(Foam *foams, const int2 *bt, uint32_t flags) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (F->d.halt) return;
+    const int n = F->d.count, ipt = F->ipt;
+    const double h = 0.5 * F->d.dt;
+    const Roles R = F->r;
+    const int base = cx.lb * kBlock * ipt;
+    double sa = 0.0, sar = 0.0;
+    const bool gas = (flags & kGas) != 0;
+    if (3 == F->c.dim) {
+        if (gas) predict_range<3, true>(R, base, ipt, n, h, sa, sar);
+        else predict_range<3, false>(R, base, ipt, n, h, sa, sar);
+    } else {
+        if (gas) predict_range<2, true>(R, base, ipt, n, h, sa, sar);
+        else predict_range<2, false>(R, base, ipt, n, h, sa, sar);
+    }
+    sa = block_sum(sa);
+    sar = block_sum(sar);
+    if (threadIdx.x == 0) {
+        F->s.part[cx.lb * kPartD + 0] = sa;
+        F->s.part[cx.lb * kPartD + 1] = sar;
+    }
+}
+
+// ---------------------------------------------------------------- pair (FP64)
+template <int D, bool GAS>
+__device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, const Scratch &S,
+                                           size_t cap, int base, int ipt, int n, double &soa,
+                                           double &soar) {
+    const double *__restrict__ px = R.xp[0];
+    const double *__restrict__ py = R.xp[1];
+    const double *__restrict__ pz = R.xp[2];
+    const double *__restrict__ pr = R.xp[3];
+    const int *__restrict__ nnA = S.nn;
+    const int *__restrict__ nbr = S.nbr;
+    const double f0 = c.f0;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        const double xi = px[i], yi = py[i], zi = D == 3 ? pz[i] : 0.0, ri = pr[i];
+        const double inv_ri = 1.0 / ri;
+        double acc[3] = {0.0, 0.0, 0.0};
+        double ovl = 0.0, dr = 0.0;
+        const int m_nn = nnA[i];
+        for (int q = 0; q < m_nn; ++q) {
+            const int j = nbr[(size_t)q * cap + i];
+            const double rj = pr[j];
+            const double r1 = i < j ? ri : rj;
+            const double r2 = i < j ? rj : ri;
+            const double radii = r1 + r2;
+            double d[3];
+            wrapped_diff<D>(c, xi, yi, zi, px[j], py[j], D == 3 ? pz[j] : 0.0, d);
+            const double m = sq_len(d);
+            if (radii * radii >= m) {
+                const double s = 1.0 / sqrt(m) - 1.0 / radii;
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] += d[a] * f0 * s;
+                if (GAS) {
+                    const double r1sq = r1 * r1;
+                    const double r2sq = r2 * r2;
+                    double oa = 0;
+                    if (m < r1sq || m < r2sq) {
+                        oa = r1sq < r2sq ? r1sq : r2sq;
+                    } else {
+                        oa = 0.5 * (r2sq - r1sq + m);
+                        oa *= oa;
+                        oa /= m;
+                        oa = r2sq - oa;
+                        oa = oa < 0 ? -oa : oa;
+                    }
+                    if (3 == D) {
+                        oa *= kPi;
+                    } else {
+                        oa = 2.0 * sqrt(oa);
+                    }
+                    ovl += oa;
+                    dr += oa * (1.0 / rj - inv_ri);
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) R.vp[a][i] = acc[a];
+        R.vp[3][i] = dr;
+        S.ov[i] = ovl;
+        soa += ovl;
+        soar += ovl * inv_ri;
+    }
+}
+
+// Reduce the per-block partials [q0, q0 + nq) of a foam in fixed order.
+__device__ __forceinline__ double reduce_parts_sum(const double *part, int nb, int q) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) a += part[b * kPartD + q];
+    return block_sum(a);
+}
+
+__device__ __forceinline__ double reduce_parts_max(const double *part, int nb, int q) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) a = fmax(a, part[b * kPartD + q]);
+    return block_max(a);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pair(Foam *foams, const int2 *bt, uint32_t flags) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (F->d.halt) return;
+    const int n = F->d.count, ipt = F->ipt;
+    const Consts c = F->c;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const size_t cap = (size_t)F->cap;
+    const int base = cx.lb * kBlock * ipt;
+    double soa = 0.0, soar = 0.0;
+    const bool gas = (flags & kGas) != 0;
+    if (3 == c.dim) {
+        if (gas) pair_range<3, true>(c, R, S, cap, base, ipt, n, soa, soar);
+        else pair_range<3, false>(c, R, S, cap, base, ipt, n, soa, soar);
+    } else {
+        if (gas) pair_range<2, true>(c, R, S, cap, base, ipt, n, soa, soar);
+        else pair_range<2, false>(c, R, S, cap, base, ipt, n, soa, soar);
+    }
+    soa = block_sum(soa);
+    soar = block_sum(soar);
+    if (threadIdx.x == 0) {
+        S.part[cx.lb * kPartD + 2] = soa;
+        S.part[cx.lb * kPartD + 3] = soar;
+    }
+    if (last_block(&F->d.ticket[kTkPair], F->nblk)) {
+        const int nb = F->nblk;
+        const double sa = reduce_parts_sum(S.part, nb, 0);
+        const double sar = reduce_parts_sum(S.part, nb, 1);
+        const double s_oa = reduce_parts_sum(S.part, nb, 2);
+        const double s_oar = reduce_parts_sum(S.part, nb, 3);
+        if (threadIdx.x == 0) {
+            F->d.s_a = sa;
+            F->d.s_ar = sar;
+            F->d.s_oa = s_oa;
+            F->d.s_oar = s_oar;
+            F->d.ticket[kTkPair] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- corrector
+// reference src/kernels.cu:546-593 (addWallVelocity)
+template <int D>
+__device__ __forceinline__ void add_wall_velocity(const Consts &c, double xp, double yp, double zp,
+                                                  double rad, double &vx, double &vy, double &vz) {
+    const double drag = 1.0 - c.wall_drag;
+    double xDrag = 1.0, velocity = 0.0;
+    auto touches = [&](double x, double low, double high) -> bool {
+        double d1 = x - low;
+        double d2 = x - high;
+        d1 = d1 * d1 < d2 * d2 ? d1 : d2;
+        if (rad * rad >= d1 * d1) {
+            d2 = d1 < 0.0 ? -1.0 : 1.0;
+            d1 *= d2;
+            velocity = d2 * c.f0 * (1.0 - d1 / rad);
+            return true;
+        }
+        return false;
+    };
+    double yDrag = 1.0;
+    velocity = 0.0;
+    if (c.wall[0] && touches(xp, c.lbb[0], c.tfr[0])) {
+        vx += velocity;
+        vy *= drag;
+        if (3 == D) vz *= drag;
+        xDrag = drag;
+    }
+    velocity = 0.0;
+    if (c.wall[1] && touches(yp, c.lbb[1], c.tfr[1])) {
+        vx *= drag;
+        vy += velocity * xDrag;
+        if (3 == D) vz *= drag;
+        yDrag = drag;
+    }
+    velocity = 0.0;
+    if (3 == D && c.wall[2] && touches(zp, c.lbb[2], c.tfr[2])) {
+        vx *= drag;
+        vy *= drag;
+        vz += velocity * xDrag * yDrag;
+    }
+}
+
+struct CorrAcc {
+    double s_v, max_err, max_r, max_exp;
+    int n_del;
+};
+
+template <int D, bool GAS, bool PATH>
+__device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, const Scratch &S,
+                                              const Dyn &dyn, int base, int ipt, int n,
+                                              CorrAcc &a) {
+    const double h = 0.5 * dyn.dt;
+    const bool walls = c.wall[0] || c.wall[1] || (3 == D && c.wall[2]);
+    const double inv_rho = (dyn.s_ar - dyn.s_oar) / (dyn.s_a - dyn.s_oa);
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        double vp[3];
+#pragma unroll
+        for (int q = 0; q < D; ++q) vp[q] = R.vp[q][i];
+        if (walls) {
+            double vz = 3 == D ? vp[2] : 0.0;
+            add_wall_velocity<D>(c, R.xp[0][i], R.xp[1][i], 3 == D ? R.xp[2][i] : 0.0, R.xp[3][i],
+                                 vp[0], vp[1], vz);
+            if (3 == D) vp[2] = vz;
+#pragma unroll
+            for (int q = 0; q < D; ++q) R.vp[q][i] = vp[q];
+        }
+        double e = 0.0, dist = 0.0;
+        double xo[3], xc[3];
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            const double pred = R.xp[q][i];
+            xo[q] = R.x[q][i];
+            const double corr = xo[q] + h * (R.v[q][i] + vp[q]);
+            xc[q] = corr;
+            R.xp[q][i] = corr;
+            e = fmax(e, fabs(pred - corr));
+            const double t = corr - R.saved[q][i];
+            dist += t * t;
+        }
+        dist = sqrt(dist);
+        int dele = 0;
+        if (GAS) {
+            double r = R.xp[3][i];
+            double area = 2.0 * kPi * r;
+            if (3 == D) area *= 2.0 * r;
+            double vr = inv_rho - 1.0 / r;
+            vr *= c.kappa * c.avg_area_in * n / dyn.s_a * (area - S.ov[i]);
+            vr += R.vp[3][i];
+            vr *= c.k_param / area;
+            R.vp[3][i] = vr;
+            const double pred = r;
+            const double corr = R.x[3][i] + h * (R.v[3][i] + vr);
+            R.xp[3][i] = corr;
+            e = fmax(e, fabs(pred - corr));
+            dist += corr - R.saved[3][i];
+            r = corr;
+            if (r > c.min_rad) {
+                double vol = r * r;
+                if (3 == D) vol *= r;
+                a.s_v += vol;
+            } else {
+                dele = 1;
+                a.n_del += 1;
+            }
+            a.max_r = fmax(a.max_r, r);
+        }
+        S.del[i] = (uint8_t)dele;
+        a.max_exp = fmax(a.max_exp, dist);
+        if (PATH) {
+            double t = xo[0] - xc[0];
+            double pd = t * t;
+            t = xo[1] - xc[1];
+            pd += t * t;
+            if (3 == D) {
+                t = xo[2] - xc[2];
+                pd += t * t;
+            }
+            R.path_new[i] = R.path[i] + sqrt(pd);
+        }
+        R.err[i] = fmax(e, R.err[i]);
+        a.max_err = fmax(a.max_err, e);
+    }
+}
+
+// dt controller and commit: reference src/simulator.cu:239-246, :258-321.
+__device__ void control(Foam *F, uint32_t flags, const CorrAcc &a) {
+    Dyn &d = F->d;
+    const Consts &c = F->c;
+    d.s_v = a.s_v;
+    d.max_err = a.max_err;
+    d.max_r = a.max_r;
+    d.max_exp = a.max_exp;
+    d.n_del = a.n_del;
+    d.attempts += 1;
+    const double dt = d.dt;
+    const bool too_large = a.max_err > c.tol;
+    const bool increase = a.max_err < 0.45 * c.tol && dt < 10;
+    double ndt = dt;
+    if (too_large) {
+        ndt = dt * 0.37;
+    } else if (increase) {
+        ndt = dt * 1.269;
+    }
+    d.dt = ndt;
+    d.accepted = too_large ? 0 : 1;
+    if (too_large || !(flags & kCommit)) return;
+    const bool gas = (flags & kGas) != 0;
+    Roles &R = F->r;
+    for (int k = 0; k < 4; ++k) {
+        if (2 == k && 2 == c.dim) continue;
+        if (3 == k && !gas) continue;
+        double *t = R.vo[k];
+        R.vo[k] = R.v[k];
+        R.v[k] = R.vp[k];
+        R.vp[k] = t;
+        t = R.x[k];
+        R.x[k] = R.xp[k];
+        R.xp[k] = t;
+    }
+    if (flags & kPath) {
+        double *t = R.path;
+        R.path = R.path_new;
+        R.path_new = t;
+    }
+    d.steps += 1;
+    d.bubble_steps += (double)d.count;
+    d.time_frac += ndt;
+    const uint64_t whole = (uint64_t)d.time_frac;
+    d.time_int += whole;
+    d.time_frac = d.time_frac - (double)whole;
+    if (gas) {
+        d.max_r_cell = a.max_r;
+        d.need_remove = a.n_del > 0 ? 1 : 0;
+    }
+    d.need_search = a.max_exp >= 0.5 * c.skin ? 1 : 0;
+    if (d.step_limit > 0 && d.steps >= d.step_limit) d.halt = 1;
+}
+
+template <int D>
+__device__ __forceinline__ void correct_dispatch(const Consts &c, const Roles &R, const Scratch &S,
+                                                 const Dyn &dyn, int base, int ipt, int n,
+                                                 uint32_t flags, CorrAcc &a) {
+    const bool gas = (flags & kGas) != 0, path = (flags & kPath) != 0;
+    if (gas) {
+        if (path) correct_range<D, true, true>(c, R, S, dyn, base, ipt, n, a);
+        else correct_range<D, true, false>(c, R, S, dyn, base, ipt, n, a);
+    } else {
+        if (path) correct_range<D, false, true>(c, R, S, dyn, base, ipt, n, a);
+        else correct_range<D, false, false>(c, R, S, dyn, base, ipt, n, a);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt, uint32_t flags) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (F->d.halt) return;
+    const int n = F->d.count, ipt = F->ipt;
+    const Consts c = F->c;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const Dyn dyn = F->d;
+    const int base = cx.lb * kBlock * ipt;
+    CorrAcc a{0.0, 0.0, 0.0, 0.0, 0};
+    if (3 == c.dim) correct_dispatch<3>(c, R, S, dyn, base, ipt, n, flags, a);
+    else correct_dispatch<2>(c, R, S, dyn, base, ipt, n, flags, a);
+    const double s_v = block_sum(a.s_v);
+    const double me = block_max(a.max_err);
+    const double mr = block_max(a.max_r);
+    const double mx = block_max(a.max_exp);
+    const int nd = block_sum_i(a.n_del);
+    if (threadIdx.x == 0) {
+        S.part[cx.lb * kPartD + 4] = s_v;
+        S.part[cx.lb * kPartD + 5] = me;
+        S.part[cx.lb * kPartD + 6] = mr;
+        S.part[cx.lb * kPartD + 7] = mx;
+        S.part_i[cx.lb * kPartI + 0] = nd;
+    }
+    if (last_block(&F->d.ticket[kTkCorr], F->nblk)) {
+        const int nb = F->nblk;
+        CorrAcc t;
+        t.s_v = reduce_parts_sum(S.part, nb, 4);
+        t.max_err = reduce_parts_max(S.part, nb, 5);
+        t.max_r = reduce_parts_max(S.part, nb, 6);
+        t.max_exp = reduce_parts_max(S.part, nb, 7);
+        int ndl = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) ndl += S.part_i[b * kPartI + 0];
+        t.n_del = block_sum_i(ndl);
+        if (threadIdx.x == 0) {
+            control(F, flags, t);
+            F->d.ticket[kTkCorr] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launches
+void launch_predict(const Grid &g, uint32_t flags) {
+    hipLaunchKernelGGL(k_predict, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub,
+                       flags);
+}
+
+void launch_pair(const Grid &g, uint32_t flags) {
+    hipLaunchKernelGGL(k_pair, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub, flags);
+}
+
+void launch_correct(const Grid &g, uint32_t flags) {
+    hipLaunchKernelGGL(k_correct, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub,
+                       flags);
+}
+
+}  // namespace cubble_amd

@@ -1,0 +1,237 @@
+/*
+ * cubble_amd.h -- C ABI of the MI355X-native cubble per-timestep path.
+ *
+ * The reference (KJLankinen/cubble) exports exactly one linkable symbol,
+ * `int32_t cubble::run(int32_t argc, char **argv)` (src/simulator.cu:1160-1187),
+ * and keeps its step interface -- step(), integrate(), searchNeighbors(),
+ * removeBubbles() -- in an anonymous namespace (src/simulator.cu:36-337).
+ * This header exposes that same interface as plain C: an opaque simulation
+ * handle owning device memory and a HIP stream, plain pointers and sizes,
+ * integer status codes, no exceptions and no torch types.
+ *
+ * Threading: one host thread per handle at a time.  Separate handles may be
+ * driven concurrently from separate threads; each owns its own stream.
+ *
+ * Array slot convention used by every state view: component 0 = x, 1 = y,
+ * 2 = z, 3 = r.  Per-bubble arrays are `count` long; z arrays are ignored
+ * (and may be NULL) in 2D.  Any NULL pointer in a view is skipped.
+ */
+#ifndef CUBBLE_AMD_H
+#define CUBBLE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CUBBLE_OK 0
+#define CUBBLE_E_INVALID (-1)     /* bad argument or configuration            */
+#define CUBBLE_E_HIP (-2)         /* HIP runtime error                         */
+#define CUBBLE_E_NOMEM (-3)       /* device memory exhausted                   */
+#define CUBBLE_E_IO (-4)          /* input file missing or malformed           */
+#define CUBBLE_E_UNSUPPORTED (-5) /* feature outside this build (imposed flow) */
+#define CUBBLE_E_CAPACITY (-6)    /* a capacity limit was exceeded             */
+#define CUBBLE_E_STATE (-7)       /* call not valid in the handle's state      */
+
+/* ---- precision modes ----------------------------------------------------- */
+#define CUBBLE_PREC_FP64 0  /* everything in FP64                               */
+#define CUBBLE_PREC_MIXED 1 /* pair forces and gas fluxes in FP32 on relative
+                               coordinates; positions, radii, error estimate and
+                               volume fix in FP64                                */
+
+typedef struct cubble_sim cubble_sim;
+
+/* Mirrors the reference's IntegrationParams flags
+ * (src/data_definitions.h:227-236). use_flow must be 0: imposed flow is not
+ * part of this build. */
+typedef struct cubble_integrate_params {
+    int32_t use_gas_exchange;
+    int32_t use_flow;
+    int32_t increment_path;
+} cubble_integrate_params;
+
+/* Mirrors the outputs the reference's step() leaves in IntegrationParams and
+ * HostData (src/simulator.cu:219-246, :303-321). */
+typedef struct cubble_step_result {
+    int32_t error_too_large;    /* last attempt rejected (ip.errorTooLarge)  */
+    int32_t num_to_be_deleted;  /* bubbles marked r <= minRad in last attempt */
+    int32_t count;              /* live bubbles                               */
+    int32_t num_pairs;          /* half-list pairs (i < j)                    */
+    int32_t need_search;        /* last accepted attempt triggered a search   */
+    int32_t halted;             /* device loop stopped (limit or capacity)    */
+    double max_error;
+    double max_radius;
+    double max_expansion;
+    double dt;                  /* time step for the next attempt             */
+    double time;                /* timeInteger + timeFraction                 */
+    int64_t steps;              /* accepted integrate() calls                 */
+    int64_t attempts;           /* step() calls                               */
+    int64_t searches;
+    int64_t removals;
+    double bubble_steps;        /* sum of live count over accepted steps      */
+} cubble_step_result;
+
+/* Programmatic configuration of a synthetic initial foam.  Field meanings
+ * follow input_parameters.json (reference input_parameters.json:1-130) plus
+ * the initial-distribution keys this build adds. */
+typedef struct cubble_config {
+    int32_t dim;              /* box.dimensionality: 2 or 3                  */
+    int32_t num_start;        /* bubbles.numStart                            */
+    int32_t num_end;          /* bubbles.numEnd (CLI stop rule)              */
+    int32_t radius_dist;      /* 0 = normal clipped to [r_lo, r_hi],
+                                 1 = log-normal (median r_mean, sigma r_std) */
+    double r_mean;            /* bubbles.radius.mean                         */
+    double r_std;             /* bubbles.radius.std                          */
+    double r_lo, r_hi;        /* clip bounds for the normal distribution     */
+    double rel_dim[3];        /* box.relativeDimensions                      */
+    int32_t wall[3];          /* box.wall.{x,y,z}                            */
+    double wall_drag;         /* box.wall.drag                               */
+    double phi, mu, sigma, kappa, k_param; /* constants.*.value              */
+    double error_tolerance;   /* errorTolerance.value                        */
+    double dt0;               /* initial time step (reference: 1e-4)         */
+    uint64_t seed;            /* rngSeed.value                               */
+    int32_t precision;        /* CUBBLE_PREC_*                               */
+    int32_t nbr_cap;          /* initial neighbour capacity per bubble (0 = default) */
+    double cell_cap_factor;   /* cell capacity / initial cells (0 = default 2) */
+    int64_t max_steps;        /* CLI stop rule: accepted steps (0 = no limit) */
+} cubble_config;
+
+/* State view: caller-owned host buffers, filled by cubble_get_state or read
+ * by cubble_set_state.  Neighbour lists are CSR, full (both directions),
+ * ascending per bubble; the reference's half list is the subset j > i. */
+typedef struct cubble_state_view {
+    double *x[4];             /* positions + radius                          */
+    double *v[4];             /* dxdt .. drdt                                */
+    double *vo[4];            /* dxdto .. drdto                              */
+    double *saved[4];         /* savedX .. savedR                            */
+    double *path;
+    double *error;
+    int32_t *wrap[3];         /* wrapCountX .. Z                             */
+    int32_t *index;           /* original bubble index                       */
+    int32_t *nbr_off;         /* count + 1 offsets                           */
+    int32_t *nbr;             /* neighbour list                              */
+    int64_t nbr_cap;          /* length of nbr (get: capacity, set: length)  */
+} cubble_state_view;
+
+/* Outputs of the last step() attempt (the reference's predicted buffers). */
+typedef struct cubble_attempt_view {
+    double *xc[4];            /* corrected positions / radius                */
+    double *vp[4];            /* predicted velocities (drdt incl. gas)       */
+    double *overlap;          /* per-bubble overlap area sum                 */
+    double *path_new;
+    uint8_t *deleted;         /* 1 = marked for deletion                     */
+} cubble_attempt_view;
+
+/* Scalars of a handle (constants and dynamic state). */
+typedef struct cubble_scalars {
+    int32_t dim;
+    int32_t count;
+    int32_t capacity;
+    int32_t nbr_cap;
+    int32_t cell_cap;
+    int32_t cell_dim[3];
+    int32_t wall[3];
+    int32_t precision;
+    double lbb[3], tfr[3], interval[3];
+    double wall_drag, skin, min_rad, f0, k_param, kappa, avg_area_in, vol_mul, tol;
+    double dt;
+    uint64_t time_int;
+    double time_frac;
+    double max_r_cell;
+    int64_t steps, attempts, searches, removals;
+    double bubble_steps;
+    double s_a, s_ar, s_oa, s_oar, s_v, max_err, max_r, max_exp;
+    int32_t n_del, accepted, need_search, num_pairs;
+} cubble_scalars;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Replaces init() (src/simulator.cu:527-956) for a synthetic foam: draws the
+ * radii and positions on the host with the project RNG, uploads them, and
+ * runs the first neighbour search (src/simulator.cu:725).  Uses device 0 of
+ * the current HIP context. */
+int cubble_create(const cubble_config *cfg, cubble_sim **out);
+
+/* Same, reading the reference's input_parameters.json schema
+ * (src/simulator.cu:538-575; keys listed in input_parameters.json). */
+int cubble_create_from_json(const char *json_path, cubble_sim **out);
+
+/* Parses input_parameters.json into a config without touching the GPU. */
+int cubble_config_from_json(const char *json_path, cubble_config *out);
+
+/* Fills `cfg` with the defaults of input_parameters.json. */
+void cubble_config_default(cubble_config *cfg);
+
+/* Frees device memory and the stream (replaces end(), src/simulator.cu:514-525). */
+void cubble_destroy(cubble_sim *sim);
+
+/* Message of the last failing call on `sim` (or of the last failing create
+ * when sim is NULL).  Valid until the next call on the same handle. */
+const char *cubble_last_error(const cubble_sim *sim);
+
+/* ---- the step interface --------------------------------------------------- */
+
+/* One attempt == step() (src/simulator.cu:192-249): predictor, pairwise
+ * velocity and gas flux, corrector, dt control.  Does not commit. */
+int cubble_step(cubble_sim *sim, const cubble_integrate_params *ip, cubble_step_result *res);
+
+/* integrate() (src/simulator.cu:251-337): attempts until accepted, commit,
+ * removal of dissolved bubbles, neighbour search when the skin is half used. */
+int cubble_integrate(cubble_sim *sim, const cubble_integrate_params *ip, cubble_step_result *res);
+
+/* integrate() repeated `num_steps` times, driven entirely on the device (no
+ * host synchronisation per step).  Replaces the main loop's integrate calls
+ * (src/simulator.cu:1008-1011).  Returns when the steps are done. */
+int cubble_run_steps(cubble_sim *sim, const cubble_integrate_params *ip, int64_t num_steps,
+                     cubble_step_result *res);
+
+/* searchNeighbors() (src/simulator.cu:60-178). */
+int cubble_search_neighbors(cubble_sim *sim);
+
+/* removeBubbles() (src/simulator.cu:180-190) using the deletion marks of the
+ * last attempt. */
+int cubble_remove_bubbles(cubble_sim *sim);
+
+/* ---- state access (tests, checkpoints) ----------------------------------- */
+int cubble_get_scalars(cubble_sim *sim, cubble_scalars *out);
+int cubble_get_state(cubble_sim *sim, cubble_state_view *view);
+int cubble_get_attempt(cubble_sim *sim, cubble_attempt_view *view);
+/* Replaces the whole state (count, arrays, scalars dt/time/max_r_cell/counters). */
+int cubble_set_state(cubble_sim *sim, const cubble_state_view *view, const cubble_scalars *sc);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* Average device time of each kernel of one attempt, measured with HIP
+ * events recorded around each launch on the handle's stream over `reps`
+ * non-committing attempts (predictor, pair, corrector) from the current
+ * state.  No counterpart in the reference (it has NVTX ranges only,
+ * src/simulator.cu:40-363); used by the benchmark's roofline report. */
+typedef struct cubble_kernel_times {
+    double predict_ms;
+    double pair_ms;
+    double correct_ms;
+    int32_t reps;
+} cubble_kernel_times;
+int cubble_time_kernels(cubble_sim *sim, const cubble_integrate_params *ip, int32_t reps,
+                        cubble_kernel_times *out);
+
+/* ---- host-only helpers (no GPU needed) ----------------------------------- */
+
+/* Synthetic initial foam on the host: the exact arrays cubble_create uploads
+ * (before the first search).  x4 holds 4 * count doubles (x, y, z, r rows);
+ * out_sc receives the derived constants (box, skin, minRad, f0, mean area,
+ * volume multiplier, cell capacity). */
+int cubble_generate_foam(const cubble_config *cfg, double *x4, cubble_scalars *out_sc);
+
+/* Drop-in command line entry == cubble::run (src/simulator.cu:1161-1186). */
+int cubble_run(int argc, char **argv);
+
+/* Version string of the library. */
+const char *cubble_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -621,7 +621,8 @@ struct Rng {
         return result;
     }
     double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
-    /* Box-Muller, both variates used in turn */
+    /* Marsaglia polar method (Marsaglia and Bray, 1964), both variates used
+     * in turn; needs no trigonometric function */
     bool have = false;
     double spare = 0.0;
     double normal() {
@@ -629,13 +630,16 @@ struct Rng {
             have = false;
             return spare;
         }
-        const double u1 = 1.0 - uniform();
-        const double u2 = uniform();
-        const double rad = std::sqrt(-2.0 * std::log(u1));
-        const double th = 2.0 * kPi * u2;
-        spare = rad * std::sin(th);
+        double u, v, s;
+        do {
+            u = 2.0 * uniform() - 1.0;
+            v = 2.0 * uniform() - 1.0;
+            s = u * u + v * v;
+        } while (s >= 1.0 || s == 0.0);
+        const double m = std::sqrt(-2.0 * std::log(s) / s);
+        spare = v * m;
         have = true;
-        return rad * std::cos(th);
+        return u * m;
     }
 };
 

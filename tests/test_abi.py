@@ -1,0 +1,90 @@
+"""CPU-only checks of the product library: it loads, exports every symbol the
+C header declares, its host-side initial foam equals the oracle's bit for bit,
+and it reads the reference's input_parameters.json schema.  No GPU compute."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cubble_amd as ca
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def header_functions():
+    text = open(ca.HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(cubble_\w+)\s*\(",
+                                 text, re.M)))
+
+
+def test_library_loads_and_exports_header_symbols():
+    L = ca.lib()
+    declared = header_functions()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(declared) == sorted(ca.EXPORTS)
+    assert "gfx950" in ca.version()
+
+
+def test_no_torch_types_in_header():
+    text = open(ca.HEADER).read()
+    for token in ("torch::", "at::", "c10::", "Tensor", "#include <torch"):
+        assert token not in text
+
+
+@pytest.mark.parametrize("dim,dist,n,phi,seed", [(2, 0, 3000, 0.9, 426), (3, 1, 4000, 0.84, 11),
+                                                 (3, 0, 1500, 0.95, 3)])
+def test_host_initial_foam_matches_oracle(ora, dim, dist, n, phi, seed):
+    r_std = 0.1 if dist == 0 else 0.15
+    cfg = ca.Config(dim=dim, num_start=n, radius_dist=dist, r_std=r_std, r_lo=0.5, r_hi=1.5,
+                    phi=phi, kappa=0.01, k_param=0.1, error_tolerance=1e-4, seed=seed)
+    x4, sc = ca.generate_foam(cfg)
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=phi, seed=seed, radius_dist=dist, r_std=r_std,
+                           kappa=0.01, k_param=0.1, tol=1e-4)
+    st, osc = f.get_state()
+    c = f.consts()
+    order = np.argsort(st["idx"])
+    for k in list(range(dim)) + [3]:
+        assert np.array_equal(st["x"][k][order], x4[k]), k
+    for key in ("interval", "tfr", "lbb"):
+        assert list(sc[key]) == list(c[key])
+    for key in ("skin", "min_rad", "f0", "k_param", "kappa", "avg_area_in", "vol_mul", "tol",
+                "cell_cap"):
+        assert sc[key] == c[key], key
+    assert sc["max_r_cell"] == osc["max_r_cell"]
+
+
+def test_json_schema_reader():
+    cfg = ca.Config.from_json(os.path.join(GOLDEN, "input_small_2d.json"))
+    assert cfg.dim == 2 and cfg.num_start == 2000 and cfg.num_end == 1000
+    assert cfg.r_mean == 1.0 and cfg.r_std == 0.1 and cfg.phi == 0.9
+    assert cfg.kappa == 0.01 and cfg.k_param == 0.1 and cfg.error_tolerance == 1e-4
+    assert cfg.seed == 426 and cfg.max_steps == 200 and cfg.radius_dist == 0
+    assert (cfg.r_lo, cfg.r_hi) == (0.5, 1.5) and tuple(cfg.wall) == (0, 0, 0)
+    c3 = ca.Config.from_json(os.path.join(GOLDEN, "input_3d_lognormal.json"))
+    assert c3.dim == 3 and c3.radius_dist == 1 and c3.precision == ca.PREC_MIXED
+    assert c3.seed == 7 and c3.phi == 0.84
+
+
+def test_json_errors_are_reported():
+    with pytest.raises(ca.CubbleError) as e:
+        ca.Config.from_json(os.path.join(GOLDEN, "input_flow.json"))
+    assert e.value.code == ca.E_UNSUPPORTED
+    with pytest.raises(ca.CubbleError) as e:
+        ca.Config.from_json(os.path.join(GOLDEN, "does_not_exist.json"))
+    assert e.value.code == ca.E_IO
+
+
+def test_invalid_config_rejected_on_host():
+    with pytest.raises(ca.CubbleError) as e:
+        ca.generate_foam(ca.Config(dim=4, num_start=10))
+    assert e.value.code == ca.E_INVALID
+    with pytest.raises(ca.CubbleError):
+        ca.generate_foam(ca.Config(dim=2, num_start=10, phi=1.5))
+
+
+def test_cli_usage_error_without_device_work():
+    # wrong argument count: returns EXIT_FAILURE before touching the GPU
+    assert ca.run_cli(["cubble-cli"]) == 1

@@ -1,0 +1,254 @@
+"""GPU parity of the HIP path against the CPU FP64 oracle (through the C ABI).
+
+Bars: neighbour lists, cell order, wrap counters, compaction: bit-exact.
+Per-step velocities and dR/dt: <= 1e-12 (scaled metric, tests/parity.py) for
+FP64, <= 1e-5 for the mixed-precision path.  Total gas volume after a volume
+fix: phi * V_box to 1e-10, and GPU vs oracle to 1e-10."""
+import numpy as np
+import pytest
+
+import cubble_amd as ca
+from parity import (FP64_TOL, MIXED_TOL, VOLUME_TOL, assert_same_structure, attempt_errors,
+                    config, oracle_for, oracle_from_sim, scaled_err, total_volume)
+
+pytestmark = pytest.mark.gpu
+
+GAS = ca.IntegrationParams(True, False, True)
+CASES = [(2, 3000, 426), (3, 4000, 7), (2, 20000, 5), (3, 60000, 9)]
+
+
+@pytest.mark.parametrize("dim,n,seed", CASES)
+def test_initial_state_and_first_search_bit_exact(ora, dim, n, seed):
+    cfg = config(dim, n, seed=seed)
+    sim = ca.Simulator(cfg)
+    f = oracle_for(ora, cfg)
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    assert gsc["count"] == osc["count"] == n
+    assert list(gsc["cell_dim"]) == list(osc["cell_dim"])
+    assert_same_structure(gst, ost)
+    for key in ("x", "saved"):
+        assert np.array_equal(gst[key][:dim], ost[key][:dim]), key
+        assert np.array_equal(gst[key][3], ost[key][3]), key
+    assert gsc["num_pairs"] == osc["num_pairs"]
+    sim.close()
+
+
+@pytest.mark.parametrize("dim,n,seed", CASES)
+def test_one_attempt_fp64(ora, dim, n, seed):
+    cfg = config(dim, n, seed=seed)
+    sim = ca.Simulator(cfg)
+    # a few committed steps so velocity history and dt are non-trivial
+    for _ in range(3):
+        sim.integrate(GAS)
+    f = oracle_from_sim(ora, sim)
+    r = sim.step(GAS)
+    f.attempt(True, True)
+    g = sim.get_attempt()
+    o = f.attempt_out()
+    errs = attempt_errors(dim, g, o, sim.scalars()["f0"])
+    print("fp64 errors", dim, n, errs, flush=True)
+    for q, e in errs.items():
+        assert e <= FP64_TOL, (q, e)
+    assert np.array_equal(g["del"], o["del"])
+    osc = f.scalars()
+    for k in ("s_a", "s_ar", "s_oa", "s_oar", "s_v"):
+        assert r is not None
+        assert abs(sim.scalars()[k] - osc[k]) <= FP64_TOL * abs(osc[k]), k
+    gsc = sim.scalars()
+    assert gsc["max_err"] == pytest.approx(osc["max_err"], rel=1e-10)
+    assert gsc["max_exp"] == pytest.approx(osc["max_exp"], rel=1e-10)
+    assert gsc["dt"] == osc["dt"]
+    sim.close()
+
+
+@pytest.mark.parametrize("dim,n,seed", [(2, 3000, 1), (3, 4000, 2)])
+def test_stepwise_integrate_from_shared_state(ora, dim, n, seed):
+    """Each step starts both sides from the GPU's state; covers accept/reject,
+    the commit rotation, searches and removals over 60 steps."""
+    cfg = config(dim, n, seed=seed)
+    sim = ca.Simulator(cfg)
+    searches = 0
+    for step in range(60):
+        f = oracle_from_sim(ora, sim)
+        sim.integrate(GAS)
+        f.integrate(True, True)
+        gst, gsc = sim.get_state()
+        ost, osc = f.get_state()
+        assert gsc["count"] == osc["count"]
+        assert gsc["dt"] == pytest.approx(osc["dt"], rel=1e-12)
+        assert gsc["steps"] == osc["steps"] and gsc["searches"] == osc["searches"]
+        assert_same_structure(gst, ost)
+        f0 = gsc["f0"]
+        assert max(scaled_err(gst["v"][k], ost["v"][k], f0) for k in range(dim)) <= FP64_TOL
+        s_r = max(float(np.max(np.abs(ost["v"][3]))), 1e-300)
+        assert scaled_err(gst["v"][3], ost["v"][3], s_r) <= FP64_TOL
+        assert np.allclose(gst["x"], ost["x"], rtol=1e-13, atol=1e-13)
+        searches = gsc["searches"]
+    assert searches >= 0
+    sim.close()
+
+
+@pytest.mark.parametrize("dim,n,seed", [(2, 5000, 3), (3, 8000, 4)])
+def test_search_with_wrap_and_reorder_bit_exact(ora, dim, n, seed):
+    cfg = config(dim, n, seed=seed)
+    sim = ca.Simulator(cfg)
+    st, sc = sim.get_state()
+    rng = np.random.default_rng(seed)
+    L = np.array(sc["interval"][:dim])
+    # jitter every bubble and push some across the periodic faces
+    st["x"][:dim] += rng.normal(0.0, 0.4, size=(dim, n))
+    st["x"][0][: n // 50] -= L[0] * 0.999
+    st["x"][1][n // 50: n // 25] += L[1] * 0.999
+    sim.set_state(st, sc)
+    f = oracle_from_sim(ora, sim)
+    sim.search_neighbors()
+    f.search()
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    assert_same_structure(gst, ost)
+    assert np.array_equal(gst["x"], ost["x"])
+    assert np.array_equal(gst["saved"], ost["saved"])
+    assert np.array_equal(gst["v"], ost["v"]) and np.array_equal(gst["vo"], ost["vo"])
+    assert list(gsc["cell_dim"]) == list(osc["cell_dim"])
+    assert gsc["num_pairs"] == osc["num_pairs"]
+    # the half list equals the O(N^2) brute-force pair set
+    if n <= 5000:
+        assert np.array_equal(ca.half_pairs(gst["nbr_off"], gst["nbr"]), f.brute_pairs())
+    sim.close()
+
+
+@pytest.mark.parametrize("dim,n,frac,seed", [(2, 4000, 0.01, 5), (3, 6000, 0.03, 6),
+                                             (3, 3000, 0.4, 8)])
+def test_removal_compaction_and_volume_fix(ora, dim, n, frac, seed):
+    cfg = config(dim, n, seed=seed)
+    sim = ca.Simulator(cfg)
+    st, sc = sim.get_state()
+    rng = np.random.default_rng(seed)
+    kill = rng.random(n) < frac
+    st["x"][3][kill] = 0.05
+    sim.set_state(st, sc)
+    f = oracle_from_sim(ora, sim)
+    sim.step(GAS)
+    f.attempt(True, True)
+    g = sim.get_attempt()
+    o = f.attempt_out()
+    assert np.array_equal(g["del"], o["del"]) and g["del"].sum() >= kill.sum()
+    sim.remove_bubbles()
+    f.remove()
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    assert gsc["count"] == osc["count"] == n - int(g["del"].sum())
+    assert_same_structure(gst, ost)
+    assert np.allclose(gst["x"][3], ost["x"][3], rtol=1e-14, atol=0)
+    for key in ("v", "vo", "saved"):
+        assert np.array_equal(gst[key], ost[key]), key
+    sim.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_volume_conserved_after_removal(ora, dim):
+    n = 5000
+    cfg = config(dim, n, seed=12)
+    sim = ca.Simulator(cfg)
+    st, sc = sim.get_state()
+    st["x"][3][::53] = 0.03
+    sim.set_state(st, sc)
+    f = oracle_from_sim(ora, sim)
+    res = sim.integrate(GAS)
+    f.integrate(True, True)
+    assert res["removals"] == 1
+    gst, gsc = sim.get_state()
+    ost, _ = f.get_state()
+    vbox = float(np.prod(gsc["interval"][:dim]))
+    vg = total_volume(dim, gst["x"][3])
+    vo = total_volume(dim, ost["x"][3])
+    assert abs(vg - cfg.phi * vbox) / (cfg.phi * vbox) <= VOLUME_TOL
+    assert abs(vg - vo) / vo <= VOLUME_TOL
+    sim.close()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 6000), (3, 5000)])
+def test_device_loop_equals_host_loop(dim, n):
+    """cubble_run_steps (graph-captured, device-driven) gives bit-identical
+    results to repeated cubble_integrate; the graph is replayed many times."""
+    cfg = config(dim, n, seed=21)
+    a = ca.Simulator(cfg)
+    b = ca.Simulator(cfg)
+    ra = a.run_steps(40, GAS)
+    for _ in range(40):
+        rb = b.integrate(GAS)
+    assert ra["steps"] == rb["steps"] == 40
+    assert ra["attempts"] == rb["attempts"]
+    sa, sca = a.get_state()
+    sb, scb = b.get_state()
+    for key in sa:
+        assert np.array_equal(sa[key], sb[key]), key
+    assert sca["dt"] == scb["dt"] and sca["bubble_steps"] == scb["bubble_steps"]
+    # a second replay batch continues identically
+    ra = a.run_steps(25, GAS)
+    for _ in range(25):
+        b.integrate(GAS)
+    sa, _ = a.get_state()
+    sb, _ = b.get_state()
+    assert np.array_equal(sa["x"], sb["x"]) and np.array_equal(sa["nbr"], sb["nbr"])
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("dim,n,seed", [(2, 5000, 31), (3, 8000, 32), (3, 60000, 33)])
+def test_mixed_precision_attempt_error(ora, dim, n, seed):
+    cfg = config(dim, n, seed=seed, precision=ca.PREC_MIXED)
+    sim = ca.Simulator(cfg)
+    for _ in range(3):
+        sim.integrate(GAS)
+    f = oracle_from_sim(ora, sim)
+    sim.step(GAS)
+    f.attempt(True, True)
+    errs = attempt_errors(dim, sim.get_attempt(), f.attempt_out(), sim.scalars()["f0"])
+    print("mixed errors", dim, n, errs, flush=True)
+    for q, e in errs.items():
+        assert e <= MIXED_TOL, (q, e)
+    sim.close()
+
+
+def test_large_foam_multiblock_partials(ora):
+    """More than 256 * 4096 bubbles: several bubbles per thread, the full
+    reduction tree and the XCD block remap."""
+    n = 1_200_000
+    cfg = config(3, n, seed=77)
+    sim = ca.Simulator(cfg)
+    f = oracle_for(ora, cfg)
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    assert_same_structure(gst, ost)
+    f2 = oracle_from_sim(ora, sim)
+    sim.step(GAS)
+    f2.attempt(True, True)
+    errs = attempt_errors(3, sim.get_attempt(), f2.attempt_out(), gsc["f0"])
+    print("large fp64 errors", errs, flush=True)
+    for q, e in errs.items():
+        assert e <= FP64_TOL, (q, e)
+    sim.close()
+
+
+def test_neighbour_capacity_growth(ora):
+    """A tiny initial list capacity forces the overflow path: lists are
+    rebuilt with a larger capacity and stay bit-exact."""
+    cfg = config(3, 4000, seed=41, nbr_cap=2)
+    sim = ca.Simulator(cfg)
+    f = oracle_for(ora, cfg)
+    gst, gsc = sim.get_state()
+    ost, _ = f.get_state()
+    assert gsc["nbr_cap"] > 2
+    assert_same_structure(gst, ost)
+    sim.close()
+
+
+def test_json_create_and_cli_run(tmp_path):
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "input_small_2d.json")
+    sim = ca.Simulator(json_path=path)
+    assert sim.scalars()["count"] == 2000
+    sim.close()
+    assert ca.run_cli(["cubble-cli", path]) == 0
