@@ -1,8 +1,13 @@
 // dev_util.hpp -- device helpers shared by the kernels: block context,
 // XCD-aware block remap, deterministic wave64 / block reductions, block
-// scans, last-block-done tickets, and the geometry (minimum-image
-// difference, cell grid) whose rounding must match the FP64 restatement
-// bit for bit.  Compiled with -ffp-contract=off: no fused multiply-add.
+// scans, and the geometry (minimum-image difference, cell grid) whose
+// rounding must match the FP64 restatement bit for bit.  Compiled with
+// -ffp-contract=off: no fused multiply-add.
+//
+// No kernel communicates between its own blocks: the eight XCD L2 caches of
+// MI355X are not coherent with each other, so a cross-block hand-off inside
+// one launch needs an agent-scope release (an L2 write-back) per block.
+// Reductions across blocks happen in a following one-block-per-foam kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -133,24 +138,6 @@ __device__ __forceinline__ int block_excl_scan(int v, int &total) {
     return wo + x - v;
 }
 
-// Last-block-done ticket.  Every block of the foam's launch must call it
-// (no early exit before it unless the whole foam exits).  Returns true in
-// the block that finished last; that block sees every other block's
-// writes and must reset *ticket to 0 before it ends.
-__device__ __forceinline__ bool last_block(uint32_t *ticket, int nb) {
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t prev = atomicAdd(ticket, 1u);
-        s_last = (prev == (uint32_t)(nb - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = s_last != 0;
-    if (last) __threadfence();
-    return last;
-}
-
 // ---- geometry (bit-identical to the FP64 restatement) --------------------
 
 // Minimum-image difference a - b: reference src/kernels.cu:1094-1114.
@@ -195,19 +182,6 @@ __device__ __forceinline__ void cell_dims(const Consts &c, double max_r, int cap
             if (cd[k] > cd[big]) big = k;
         cd[big] -= 1;
     }
-}
-
-// Cell of a position: reference src/kernels.cu:1191-1208, each axis index
-// clamped into [0, cellDim).
-__device__ __forceinline__ int cell_of_pos(const Consts &c, const int cd[3], double x, double y,
-                                           double z) {
-    const double p[3] = {x, y, z};
-    int id[3] = {0, 0, 0};
-    for (int k = 0; k < c.dim; ++k) {
-        const double q = floor(cd[k] * (p[k] - c.lbb[k]) / c.interval[k]);
-        id[k] = q < 0.0 ? 0 : (q >= (double)cd[k] ? cd[k] - 1 : (int)q);
-    }
-    return id[2] * cd[0] * cd[1] + id[1] * cd[0] + id[0];
 }
 
 // Newton cube root from correctly rounded operations only (frexp / ldexp

@@ -17,7 +17,7 @@
 namespace cubble_amd {
 
 constexpr int kBlock = 256;         // threads per block, bubble- and cell-space
-constexpr int kMaxBlocks = 4096;    // bubble-space blocks per foam (partials)
+constexpr int kMaxBlocks = 2048;    // bubble-space blocks per foam (partials)
 constexpr int kNumXcd = 8;          // MI355X: 8 XCDs, workgroups round-robin
 constexpr int kPartD = 8;           // doubles per block partial record
 constexpr int kPartI = 4;           // ints per block partial record
@@ -47,22 +47,28 @@ struct Roles {
     int32_t *idx, *idx_alt;
 };
 
+// One bubble's position and radius as a 32-byte record: a neighbour gather
+// touches one cache line instead of four.
+struct alignas(32) D4 {
+    double x, y, z, r;
+};
+
 struct Scratch {
+    D4 *p4;              // predicted x, y, z, r of the current attempt (AoS)
+    D4 *x4;              // current x, y, z, r in cell order, written by the search
     double *ov;          // per-bubble overlap area sum
-    double *invr;        // per-bubble 1 / r_predicted (mixed-precision path)
-    float *xf;           // float4 per bubble: predicted position relative to saved + r
     uint8_t *del;        // deletion mark of the last attempt
     int32_t *nn;         // neighbour count
     int32_t *nbr;        // neighbour lists, column-major [nbr_cap][cap]
     int32_t *cell_of;    // cell of each bubble (old order)
-    int32_t *newcell;    // cell of each bubble (new order)
     int32_t *cell_count;
     int32_t *cell_start; // num_cells + 1
     int32_t *cell_cursor;
     int32_t *members;    // unsorted cell segments
     double *part;        // bubble-space block partials [kMaxBlocks][kPartD]
     int32_t *part_i;     // bubble-space block partials [kMaxBlocks][kPartI]
-    int32_t *cpart;      // cell-space block partials (sums, then offsets)
+    int32_t *cpart;      // cell-space block partials (cell-count sums)
+    int32_t *cpart_i;    // neighbour-search block partials [sblk][kPartI]
     int32_t *del_list;   // sorted deleted indices
     int32_t *plan_src;   // source (tail survivor) of hole k
     int32_t *moved_to;   // new slot of tail survivor (old index - new count)
@@ -91,7 +97,6 @@ struct Dyn {
     double bubble_steps;
     double s_a, s_ar, s_oa, s_oar, s_v, max_err, max_r, max_exp;
     double vol_fix;      // multiplier of the pending volume fix
-    uint32_t ticket[8];  // last-block-done counters, one per kernel kind
 };
 
 struct Foam {
@@ -106,12 +111,11 @@ struct Foam {
     int32_t ipt;         // bubbles per thread in bubble-space kernels
     int32_t cblk;        // cell-space blocks
     int32_t cipt;        // cells per thread in cell-space kernels
+    int32_t sblk;        // neighbour-search blocks (kSearchCells cells each)
     int32_t precision;
 };
 
-// ticket slots
-enum Ticket { kTkPair = 0, kTkCorr = 1, kTkRmEmit = 2, kTkRmApply = 3, kTkCells = 4,
-              kTkScan = 5, kTkReorder = 6, kTkSearch = 7 };
+constexpr int kSearchCells = 32;  // cells per neighbour-search block (8 per wave)
 
 // launch flags
 enum Flags : uint32_t {

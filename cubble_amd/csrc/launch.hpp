@@ -15,6 +15,8 @@ struct Grid {
     int nb_bub;
     const int2 *bt_cell;
     int nb_cell;
+    const int2 *bt_scell;  // neighbour-search grid (kSearchCells cells per block)
+    int nb_scell;
     int nfoams;
     hipStream_t stream;
 };
@@ -22,6 +24,9 @@ struct Grid {
 void launch_predict(const Grid &g, uint32_t flags);
 void launch_pair(const Grid &g, uint32_t flags);
 void launch_pair_mixed(const Grid &g, uint32_t flags);
+// S_A, S_Ar, S_OA, S_OAr from the predictor and pair partials (one block per foam)
+void launch_sums(const Grid &g);
+// corrector, then the dt controller / commit (one block per foam)
 void launch_correct(const Grid &g, uint32_t flags);
 
 // removeBubbles: emit the sorted deletion list and plan the moves, then move

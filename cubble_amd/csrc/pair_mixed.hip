@@ -19,10 +19,7 @@ template <int D, bool GAS>
 __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R,
                                                  const Scratch &S, size_t cap, int base, int ipt,
                                                  int n, double &soa, double &soar) {
-    const double *__restrict__ px = R.xp[0];
-    const double *__restrict__ py = R.xp[1];
-    const double *__restrict__ pz = R.xp[2];
-    const double *__restrict__ pr = R.xp[3];
+    const D4 *__restrict__ p4 = S.p4;
     const int *__restrict__ nnA = S.nn;
     const int *__restrict__ nbr = S.nbr;
     const float f0 = (float)c.f0;
@@ -38,7 +35,8 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const double xi = px[i], yi = py[i], zi = D == 3 ? pz[i] : 0.0, ri = pr[i];
+        const D4 pi = p4[i];
+        const double xi = pi.x, yi = pi.y, zi = pi.z, ri = pi.r;
         const float ri_f = (float)ri;
         const float inv_ri = __frcp_rn(ri_f);
         float acc[3] = {0.f, 0.f, 0.f};
@@ -46,8 +44,9 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
         const int m_nn = nnA[i];
         for (int q = 0; q < m_nn; ++q) {
             const int j = nbr[(size_t)q * cap + i];
-            const double rj = pr[j];
-            double dd[3] = {xi - px[j], yi - py[j], D == 3 ? zi - pz[j] : 0.0};
+            const D4 pj = p4[j];
+            const double rj = pj.r;
+            double dd[3] = {xi - pj.x, yi - pj.y, D == 3 ? zi - pj.z : 0.0};
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 if (per[a] && fabs(dd[a]) > half[a]) dd[a] -= dd[a] < 0 ? -L[a] : L[a];
@@ -89,12 +88,6 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
     }
 }
 
-__device__ __forceinline__ double mixed_parts_sum(const double *part, int nb, int q) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) a += part[b * kPartD + q];
-    return block_sum(a);
-}
-
 __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *bt,
                                                        uint32_t flags) {
     const Ctx cx = bubble_ctx(foams, bt);
@@ -120,20 +113,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *
     if (threadIdx.x == 0) {
         S.part[cx.lb * kPartD + 2] = soa;
         S.part[cx.lb * kPartD + 3] = soar;
-    }
-    if (last_block(&F->d.ticket[kTkPair], F->nblk)) {
-        const int nb = F->nblk;
-        const double sa = mixed_parts_sum(S.part, nb, 0);
-        const double sar = mixed_parts_sum(S.part, nb, 1);
-        const double s_oa = mixed_parts_sum(S.part, nb, 2);
-        const double s_oar = mixed_parts_sum(S.part, nb, 3);
-        if (threadIdx.x == 0) {
-            F->d.s_a = sa;
-            F->d.s_ar = sar;
-            F->d.s_oa = s_oa;
-            F->d.s_oar = s_oar;
-            F->d.ticket[kTkPair] = 0;
-        }
     }
 }
 

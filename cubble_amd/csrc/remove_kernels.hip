@@ -2,16 +2,18 @@
 //
 //   k_rm_emit  : blocks holding marked bubbles write them, ascending, into the
 //                deletion list at their prefix offset (block counts come from
-//                the corrector's partials); the last block plans the moves:
-//                the k-th hole below the new count receives the k-th surviving
-//                bubble counted from the back (reference swapDataCountPairs,
-//                src/kernels.cu:775-857), and computes the volume-fix
-//                multiplier M / S_V^(1/D) (src/kernels.cu:861-867)
+//                the corrector's partials)
+//   k_rm_plan  : one block per foam plans the moves -- the k-th hole below the
+//                new count receives the k-th surviving bubble counted from the
+//                back (reference swapDataCountPairs, src/kernels.cu:775-857) --
+//                and the volume-fix multiplier M / S_V^(1/D)
+//                (src/kernels.cu:861-867)
 //   k_rm_apply : every slot below the new count pulls its state (a hole from
 //                its tail survivor), drops neighbours that were deleted, maps
 //                moved neighbours to their new slots, keeps the list
 //                ascending, and applies the volume fix (reference
 //                addVolumeFixPairs, src/kernels.cu:859-936)
+//   k_rm_fin   : one block per foam commits the new count
 //
 // The reference appends the deletion list with an atomic counter, so its
 // hole / survivor pairing depends on thread timing; a sorted list makes the
@@ -22,7 +24,31 @@
 
 namespace cubble_amd {
 
-__device__ void plan_moves(Foam *F) {
+__global__ __launch_bounds__(kBlock) void k_rm_emit(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (!F->d.need_remove) return;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int cnt = S.part_i[cx.lb * kPartI + 0];
+    if (cnt == 0) return;  // uniform per block
+    int acc = 0;
+    for (int b = threadIdx.x; b < cx.lb; b += kBlock) acc += S.part_i[b * kPartI + 0];
+    int off = block_sum_i(acc);
+    const int base = cx.lb * kBlock * ipt;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        const int f = (i < n && S.del[i]) ? 1 : 0;
+        int tot = 0;
+        const int pre = block_excl_scan(f, tot);
+        if (f) S.del_list[off + pre] = i;
+        off += tot;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rm_plan(Foam *foams) {
+    Foam *F = foams + blockIdx.x;
+    if (!F->d.need_remove) return;
     const Scratch S = F->s;
     const int n = F->d.count, nd = F->d.n_del, nnew = n - nd;
     int below = 0;
@@ -46,33 +72,6 @@ __device__ void plan_moves(Foam *F) {
         vm *= F->c.vol_mul;
         F->d.vol_fix = vm;
         F->d.n_holes = h;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_rm_emit(Foam *foams, const int2 *bt) {
-    const Ctx cx = bubble_ctx(foams, bt);
-    Foam *F = cx.F;
-    if (!F->d.need_remove) return;
-    const Scratch S = F->s;
-    const int n = F->d.count, ipt = F->ipt;
-    const int cnt = S.part_i[cx.lb * kPartI + 0];
-    if (cnt > 0) {
-        int acc = 0;
-        for (int b = threadIdx.x; b < cx.lb; b += kBlock) acc += S.part_i[b * kPartI + 0];
-        int off = block_sum_i(acc);
-        const int base = cx.lb * kBlock * ipt;
-        for (int k = 0; k < ipt; ++k) {
-            const int i = base + k * kBlock + (int)threadIdx.x;
-            const int f = (i < n && S.del[i]) ? 1 : 0;
-            int tot = 0;
-            const int pre = block_excl_scan(f, tot);
-            if (f) S.del_list[off + pre] = i;
-            off += tot;
-        }
-    }
-    if (last_block(&F->d.ticket[kTkRmEmit], F->nblk)) {
-        plan_moves(F);
-        if (threadIdx.x == 0) F->d.ticket[kTkRmEmit] = 0;
     }
 }
 
@@ -144,24 +143,28 @@ __global__ __launch_bounds__(kBlock) void k_rm_apply(Foam *foams, const int2 *bt
                                : apply_range<2>(R, S, cap, base, ipt, nnew, h, vm);
     sum_nn = block_sum_i(sum_nn);
     if (threadIdx.x == 0) S.part_i[cx.lb * kPartI + 1] = sum_nn;
-    if (last_block(&F->d.ticket[kTkRmApply], F->nblk)) {
-        const int nb = F->nblk;
-        int s = 0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) s += S.part_i[b * kPartI + 1];
-        s = block_sum_i(s);
-        if (threadIdx.x == 0) {
-            F->d.count = nnew;
-            F->d.removals += 1;
-            F->d.num_pairs = s / 2;
-            F->d.need_remove = 0;
-            F->d.ticket[kTkRmApply] = 0;
-        }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rm_fin(Foam *foams) {
+    Foam *F = foams + blockIdx.x;
+    if (!F->d.need_remove) return;
+    const int nb = F->nblk;
+    int s = 0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) s += F->s.part_i[b * kPartI + 1];
+    s = block_sum_i(s);
+    if (threadIdx.x == 0) {
+        F->d.count = F->d.count - F->d.n_del;
+        F->d.removals += 1;
+        F->d.num_pairs = s / 2;
+        F->d.need_remove = 0;
     }
 }
 
 void launch_remove(const Grid &g) {
     hipLaunchKernelGGL(k_rm_emit, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_rm_plan, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
     hipLaunchKernelGGL(k_rm_apply, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_rm_fin, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
 }
 
 }  // namespace cubble_amd

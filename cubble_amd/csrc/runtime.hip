@@ -2,7 +2,7 @@
 //
 // A `cubble_sim` owns one foam: its device buffers, its device header
 // (Foam), the block tables of its launches and a HIP stream.  The step
-// control flow runs on the device (the corrector's last block decides
+// control flow runs on the device (a one-block control kernel decides
 // accept / reject, dt, removal and search), so a run of many steps is a
 // queue of identical kernel groups, captured once into a HIP graph and
 // replayed, with one host synchronisation per replay.
@@ -38,6 +38,7 @@ struct cubble_sim {
     Foam *dfoam = nullptr; // device header
     int2 *bt_bub = nullptr;
     int2 *bt_cell = nullptr;
+    int2 *bt_scell = nullptr;
     std::vector<void *> allocs;
     Dyn *pinned = nullptr; // pinned readback of the dynamic scalars
     std::string err;
@@ -83,6 +84,8 @@ Grid grid(const cubble_sim *s) {
     g.nb_bub = s->h.nblk;
     g.bt_cell = s->bt_cell;
     g.nb_cell = s->h.cblk;
+    g.bt_scell = s->bt_scell;
+    g.nb_scell = s->h.sblk;
     g.nfoams = 1;
     g.stream = s->stream;
     return g;
@@ -194,12 +197,25 @@ int settle(cubble_sim *s, Dyn &d) {
     return CUBBLE_OK;
 }
 
-void enqueue_group(cubble_sim *s, uint32_t flags) {
-    const Grid g = grid(s);
-    launch_predict(g, flags);
+void launch_pair_any(cubble_sim *s, const Grid &g, uint32_t flags) {
     if (s->precision == CUBBLE_PREC_MIXED) launch_pair_mixed(g, flags);
     else launch_pair(g, flags);
+}
+
+// One step() attempt: predictor, pair, sums, corrector, control.
+void enqueue_attempt(cubble_sim *s, uint32_t flags) {
+    const Grid g = grid(s);
+    launch_predict(g, flags);
+    launch_pair_any(s, g, flags);
+    launch_sums(g);
     launch_correct(g, flags);
+}
+
+// One integrate() group: an attempt, then removal and search, each of which
+// runs only if the attempt was accepted and the device control asked for it.
+void enqueue_group(cubble_sim *s, uint32_t flags) {
+    const Grid g = grid(s);
+    enqueue_attempt(s, flags);
     launch_remove(g);
     launch_search(g);
 }
@@ -266,6 +282,7 @@ int make_sim(const cubble_config *cfg, cubble_sim **out) {
     F.nblk = (n + kBlock * F.ipt - 1) / (kBlock * F.ipt);
     F.cipt = std::max(1, (F.cell_cap + kBlock * 1024 - 1) / (kBlock * 1024));
     F.cblk = std::max(1, (F.cell_cap + kBlock * F.cipt - 1) / (kBlock * F.cipt));
+    F.sblk = std::max(1, (F.cell_cap + kSearchCells - 1) / kSearchCells);
     const size_t cap = (size_t)n;
     // buffers: positions x2, velocities x4, saved, path x2, err x2
     double *xb[2][4], *vb[4][4];
@@ -292,10 +309,11 @@ int make_sim(const cubble_config *cfg, cubble_sim **out) {
             return bail(rc);
     if ((rc = dalloc(s, &F.r.idx, cap)) || (rc = dalloc(s, &F.r.idx_alt, cap))) return bail(rc);
     Scratch &S = F.s;
-    if ((rc = dalloc(s, &S.ov, cap)) || (rc = dalloc(s, &S.invr, cap)) ||
-        (rc = dalloc(s, &S.xf, cap * 4)) || (rc = dalloc(s, &S.del, cap)) ||
+    if ((rc = dalloc(s, &S.p4, cap)) || (rc = dalloc(s, &S.x4, cap)) ||
+        (rc = dalloc(s, &S.ov, cap)) || (rc = dalloc(s, &S.del, cap)) ||
         (rc = dalloc(s, &S.nn, cap)) || (rc = dalloc(s, &S.nbr, cap * F.nbr_cap)) ||
-        (rc = dalloc(s, &S.cell_of, cap)) || (rc = dalloc(s, &S.newcell, cap)) ||
+        (rc = dalloc(s, &S.cell_of, cap)) ||
+        (rc = dalloc(s, &S.cpart_i, (size_t)F.sblk * kPartI)) ||
         (rc = dalloc(s, &S.cell_count, (size_t)F.cell_cap)) ||
         (rc = dalloc(s, &S.cell_start, (size_t)F.cell_cap + 1)) ||
         (rc = dalloc(s, &S.cell_cursor, (size_t)F.cell_cap)) ||
@@ -326,14 +344,18 @@ int make_sim(const cubble_config *cfg, cubble_sim **out) {
     d.max_r_cell = io.max_r;
     d.need_search = 1;
     // block tables
-    std::vector<int2> bt(F.nblk), bc(F.cblk);
+    std::vector<int2> bt(F.nblk), bc(F.cblk), bs(F.sblk);
     for (int b = 0; b < F.nblk; ++b) bt[b] = make_int2(0, b);
     for (int b = 0; b < F.cblk; ++b) bc[b] = make_int2(0, b);
-    if ((rc = dalloc(s, &s->bt_bub, bt.size())) || (rc = dalloc(s, &s->bt_cell, bc.size())))
+    for (int b = 0; b < F.sblk; ++b) bs[b] = make_int2(0, b);
+    if ((rc = dalloc(s, &s->bt_bub, bt.size())) || (rc = dalloc(s, &s->bt_cell, bc.size())) ||
+        (rc = dalloc(s, &s->bt_scell, bs.size())))
         return bail(rc);
     if (hipMemcpy(s->bt_bub, bt.data(), sizeof(int2) * bt.size(), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(s->bt_cell, bc.data(), sizeof(int2) * bc.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(s->bt_scell, bs.data(), sizeof(int2) * bs.size(), hipMemcpyHostToDevice) !=
             hipSuccess) {
         s->err = "upload of block tables failed";
         return bail(CUBBLE_E_HIP);
@@ -469,10 +491,7 @@ int cubble_step(cubble_sim *s, const cubble_integrate_params *ip, cubble_step_re
     const uint32_t flags = flags_of(ip);
     const Grid g = grid(s);
     launch_set_flags(g, -1, -1, 1);
-    launch_predict(g, flags);
-    if (s->precision == CUBBLE_PREC_MIXED) launch_pair_mixed(g, flags);
-    else launch_pair(g, flags);
-    launch_correct(g, flags);
+    enqueue_attempt(s, flags);
     HIP_TRY(s, hipGetLastError());
     Dyn d;
     rc = read_dyn(s, d);
@@ -561,9 +580,10 @@ int cubble_time_kernels(cubble_sim *s, const cubble_integrate_params *ip, int32_
         HIP_TRY(s, hipEventRecord(e[0], s->stream));
         launch_predict(g, flags);
         HIP_TRY(s, hipEventRecord(e[1], s->stream));
-        if (s->precision == CUBBLE_PREC_MIXED) launch_pair_mixed(g, flags);
-        else launch_pair(g, flags);
+        launch_pair_any(s, g, flags);
         HIP_TRY(s, hipEventRecord(e[2], s->stream));
+        // the corrector bucket includes the two one-block reduction kernels
+        launch_sums(g);
         launch_correct(g, flags);
         HIP_TRY(s, hipEventRecord(e[3], s->stream));
     }

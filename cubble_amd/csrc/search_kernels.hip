@@ -3,7 +3,7 @@
 //   k_s_cells   : wrap positions into the box on periodic axes, cell id of
 //                 every bubble, cell counts (reference wrapOverPeriodicBoundaries
 //                 src/kernels.cu:992-1011, cellByPosition :598-607)
-//   k_s_scan1   : per-block cell-count sums; the last block scans them
+//   k_s_scan1   : per-block cell-count sums
 //   k_s_scan2   : cell offsets (exclusive scan), cursors, counts reset
 //                 (reference: cub::DeviceScan::InclusiveSum, simulator.cu:90-91)
 //   k_s_bin     : bubbles into unsorted cell segments (reference indexByCell
@@ -11,15 +11,18 @@
 //   k_s_reorder : each bubble ranks itself among its cell's members by old
 //                 index (a stable counting sort, where the reference's
 //                 atomic order is arbitrary) and scatters its whole state to
-//                 its new slot; saved <- current; the last block swaps the
-//                 role pointers (reference reorganizeByIndex :721-770 and the
-//                 pointer rotation in simulator.cu:96-147)
+//                 its new slot in the alternate buffers; saved <- current
+//                 (reference reorganizeByIndex :721-770)
 //   k_s_search  : full neighbour list of every bubble, ascending, over the
-//                 de-duplicated 3^D cell neighbourhood (reference
-//                 neighborSearch / comparePair src/kernels.cu:617-701 with
-//                 getNeighborCellIndex :1116-1189); the reference's half list
-//                 is the subset j > i.  Counts beyond the list capacity are
-//                 reported, never written out of bounds.
+//                 de-duplicated 3^D cell neighbourhood, one wavefront per
+//                 cell (reference neighborSearch / comparePair
+//                 src/kernels.cu:617-701 with getNeighborCellIndex
+//                 :1116-1189); the reference's half list is the subset j > i.
+//                 Counts beyond the list capacity are reported, never written
+//                 out of bounds.
+//   k_s_fin     : one block per foam makes the reordered buffers current
+//                 (the pointer rotation of simulator.cu:96-147) and totals the
+//                 pair count
 #include "dev_util.hpp"
 #include "launch.hpp"
 
@@ -68,14 +71,12 @@ __global__ __launch_bounds__(kBlock) void k_s_cells(Foam *foams, const int2 *bt)
     const int base = cx.lb * kBlock * ipt;
     if (3 == c.dim) cells_range<3>(c, R, S, cd, base, ipt, n);
     else cells_range<2>(c, R, S, cd, base, ipt, n);
-    if (last_block(&F->d.ticket[kTkCells], F->nblk)) {
-        if (threadIdx.x == 0) {
-            F->d.cell_dim[0] = cd[0];
-            F->d.cell_dim[1] = cd[1];
-            F->d.cell_dim[2] = cd[2];
-            F->d.num_cells = cd[0] * cd[1] * cd[2];
-            F->d.ticket[kTkCells] = 0;
-        }
+    // every block computed the same grid; no block of this kernel reads it back
+    if (0 == cx.lb && 0 == threadIdx.x) {
+        F->d.cell_dim[0] = cd[0];
+        F->d.cell_dim[1] = cd[1];
+        F->d.cell_dim[2] = cd[2];
+        F->d.num_cells = cd[0] * cd[1] * cd[2];
     }
 }
 
@@ -93,27 +94,11 @@ __global__ __launch_bounds__(kBlock) void k_s_scan1(Foam *foams, const int2 *bt)
     }
     s = block_sum_i(s);
     if (threadIdx.x == 0) S.cpart[cx.lb] = s;
-    if (last_block(&F->d.ticket[kTkScan], F->cblk)) {
-        // exclusive scan of the block sums, each thread owning a contiguous run
-        const int nb = F->cblk;
-        const int per = (nb + kBlock - 1) / kBlock;
-        const int b0 = (int)threadIdx.x * per;
-        int loc = 0;
-        for (int b = b0; b < b0 + per && b < nb; ++b) loc += S.cpart[b];
-        int tot = 0;
-        int off = block_excl_scan(loc, tot);
-        for (int b = b0; b < b0 + per && b < nb; ++b) {
-            const int v = S.cpart[b];
-            S.cpart[b] = off;
-            off += v;
-        }
-        if (threadIdx.x == 0) {
-            S.cell_start[ncell] = tot;
-            F->d.ticket[kTkScan] = 0;
-        }
-    }
 }
 
+// Cell offsets: each block sums the block totals before it (at most a few
+// hundred ints), then scans its own cells.  The block holding the last cell
+// also writes start[num_cells].
 __global__ __launch_bounds__(kBlock) void k_s_scan2(Foam *foams, const int2 *bt) {
     const Ctx cx = cell_ctx(foams, bt);
     Foam *F = cx.F;
@@ -121,7 +106,10 @@ __global__ __launch_bounds__(kBlock) void k_s_scan2(Foam *foams, const int2 *bt)
     const Scratch S = F->s;
     const int ncell = F->d.num_cells, cipt = F->cipt;
     const int base = cx.lb * kBlock * cipt;
-    int off = S.cpart[cx.lb];
+    if (base >= ncell) return;  // uniform per block
+    int acc = 0;
+    for (int b = threadIdx.x; b < cx.lb; b += kBlock) acc += S.cpart[b];
+    int off = block_sum_i(acc);
     for (int k = 0; k < cipt; ++k) {
         const int q = base + k * kBlock + (int)threadIdx.x;
         const int v = q < ncell ? S.cell_count[q] : 0;
@@ -134,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_s_scan2(Foam *foams, const int2 *bt)
         }
         off += tot;
     }
+    if (ncell - 1 < base + kBlock * cipt && threadIdx.x == 0) S.cell_start[ncell] = off;
 }
 
 __global__ __launch_bounds__(kBlock) void k_s_bin(Foam *foams, const int2 *bt) {
@@ -162,21 +151,23 @@ __device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, 
         int rank = 0;
         for (int s = s0; s < s1; ++s) rank += S.members[s] < i ? 1 : 0;
         const int np = s0 + rank;
+        double xs[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (2 == q && 2 == D) continue;
             const double xv = R.x[q][i];
+            xs[q] = xv;
             R.xp[q][np] = xv;
             R.saved[q][np] = xv;
             R.vp[q][np] = R.v[q][i];
             R.vs[q][np] = R.vo[q][i];
         }
+        S.x4[np] = D4{xs[0], xs[1], xs[2], xs[3]};
         R.path_new[np] = R.path[i];
         R.err_alt[np] = R.err[i];
 #pragma unroll
         for (int q = 0; q < D; ++q) R.wc_alt[q][np] = R.wc[q][i];
         R.idx_alt[np] = R.idx[i];
-        S.newcell[np] = c;
     }
 }
 
@@ -190,37 +181,36 @@ __global__ __launch_bounds__(kBlock) void k_s_reorder(Foam *foams, const int2 *b
     const int base = cx.lb * kBlock * ipt;
     if (3 == F->c.dim) reorder_range<3>(R, S, base, ipt, n);
     else reorder_range<2>(R, S, base, ipt, n);
-    if (last_block(&F->d.ticket[kTkReorder], F->nblk)) {
-        if (threadIdx.x == 0) {
-            Roles &W = F->r;
-            for (int q = 0; q < 4; ++q) {
-                double *t = W.x[q];
-                W.x[q] = W.xp[q];
-                W.xp[q] = t;
-                t = W.v[q];
-                W.v[q] = W.vp[q];
-                W.vp[q] = t;
-                t = W.vo[q];
-                W.vo[q] = W.vs[q];
-                W.vs[q] = t;
-            }
-            double *t = W.path;
-            W.path = W.path_new;
-            W.path_new = t;
-            t = W.err;
-            W.err = W.err_alt;
-            W.err_alt = t;
-            for (int q = 0; q < 3; ++q) {
-                int *u = W.wc[q];
-                W.wc[q] = W.wc_alt[q];
-                W.wc_alt[q] = u;
-            }
-            int *u = W.idx;
-            W.idx = W.idx_alt;
-            W.idx_alt = u;
-            F->d.ticket[kTkReorder] = 0;
-        }
+}
+
+// After the reorder the new state sits in the alternate buffers; k_s_fin
+// makes them current.
+__device__ void swap_reordered_roles(Roles &W) {
+    for (int q = 0; q < 4; ++q) {
+        double *t = W.x[q];
+        W.x[q] = W.xp[q];
+        W.xp[q] = t;
+        t = W.v[q];
+        W.v[q] = W.vp[q];
+        W.vp[q] = t;
+        t = W.vo[q];
+        W.vo[q] = W.vs[q];
+        W.vs[q] = t;
     }
+    double *t = W.path;
+    W.path = W.path_new;
+    W.path_new = t;
+    t = W.err;
+    W.err = W.err_alt;
+    W.err_alt = t;
+    for (int q = 0; q < 3; ++q) {
+        int *u = W.wc[q];
+        W.wc[q] = W.wc_alt[q];
+        W.wc_alt[q] = u;
+    }
+    int *u = W.idx;
+    W.idx = W.idx_alt;
+    W.idx_alt = u;
 }
 
 // Sorted, de-duplicated neighbour coordinates of cell coordinate `x` along
@@ -248,35 +238,38 @@ __device__ __forceinline__ int axis_cells(int x, int n, bool wall, int out[3]) {
     return cnt;
 }
 
-// Full neighbour list of every bubble in ascending index order.  The linear
-// cell id z*cx*cy + y*cx + x grows lexicographically in (z, y, x), so walking
-// the sorted per-axis cell lists in that nesting visits cells -- and, since
-// bubbles are stored in cell order, candidates -- in ascending order.
-// Consecutive x-cells of one row are one contiguous range of bubbles.
+// Neighbour lists, one wavefront per cell.  The linear cell id
+// z*cx*cy + y*cx + x grows lexicographically in (z, y, x), so walking the
+// sorted per-axis cell lists in that nesting visits cells -- and, since
+// bubbles are stored in cell order, candidates -- in ascending index order;
+// consecutive x-cells of one row form one contiguous run of bubbles.  The 64
+// lanes take 64 consecutive candidates of a run (one coalesced 32-byte
+// record each); the cell's own bubbles are taken one at a time as uniform
+// (scalar) loads; hits are found with a wave ballot and appended to that
+// bubble's list in lane order, which keeps every list ascending.  Lists
+// longer than the capacity are counted, never written out of bounds.
 template <int D>
-__device__ __forceinline__ void search_range(const Consts &c, const Roles &R, const Scratch &S,
-                                             const int cd[3], size_t cap, int nbr_cap, int base,
-                                             int ipt, int n, int &sum_nn, int &max_nn, int &ovf) {
-    const double *__restrict__ px = R.x[0];
-    const double *__restrict__ py = R.x[1];
-    const double *__restrict__ pz = R.x[2];
-    const double *__restrict__ pr = R.x[3];
-    const int *__restrict__ start = S.cell_start;
+__device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4,
+                                            const int *__restrict__ start, int *__restrict__ nbr,
+                                            int *__restrict__ nn, const int cd[3], size_t cap,
+                                            int nbr_cap, int cell, int &sum_nn, int &max_nn,
+                                            int &ovf) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int i0 = start[cell], i1 = start[cell + 1];
+    if (i0 >= i1) return;  // uniform per wave
+    const int ccx = cell % cd[0];
+    const int ccy = (cell / cd[0]) % cd[1];
+    const int ccz = cell / (cd[0] * cd[1]);
+    int xs[3], ys[3], zs[3];
+    const int nx = axis_cells(ccx, cd[0], c.wall[0] != 0, xs);
+    const int ny = axis_cells(ccy, cd[1], c.wall[1] != 0, ys);
+    int nz = 1;
+    zs[0] = 0;
+    if (3 == D) nz = axis_cells(ccz, cd[2], c.wall[2] != 0, zs);
     const double skin = c.skin;
-    for (int k = 0; k < ipt; ++k) {
-        const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= n) break;
-        const int ci = S.newcell[i];
-        const int ccx = ci % cd[0];
-        const int ccy = (ci / cd[0]) % cd[1];
-        const int ccz = ci / (cd[0] * cd[1]);
-        int xs[3], ys[3], zs[3];
-        const int nx = axis_cells(ccx, cd[0], c.wall[0] != 0, xs);
-        const int ny = axis_cells(ccy, cd[1], c.wall[1] != 0, ys);
-        int nz = 1;
-        zs[0] = 0;
-        if (3 == D) nz = axis_cells(ccz, cd[2], c.wall[2] != 0, zs);
-        const double xi = px[i], yi = py[i], zi = D == 3 ? pz[i] : 0.0, ri = pr[i];
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int cb = i0; cb < i1; cb += 64) {
+        const int nb = min(64, i1 - cb);
         int cnt = 0;
         for (int a = 0; a < nz; ++a) {
             for (int b = 0; b < ny; ++b) {
@@ -286,73 +279,103 @@ __device__ __forceinline__ void search_range(const Consts &c, const Roles &R, co
                     int f = e;
                     while (f + 1 < nx && xs[f + 1] == xs[f] + 1) ++f;
                     const int j0 = start[row + xs[e]], j1 = start[row + xs[f] + 1];
-                    for (int j = j0; j < j1; ++j) {
-                        if (j == i) continue;
-                        const double maxd = ri + pr[j] + skin;
-                        double d[3];
-                        wrapped_diff<D>(c, xi, yi, zi, px[j], py[j], D == 3 ? pz[j] : 0.0, d);
-                        if (sq_len(d) < maxd * maxd) {
-                            if (cnt < nbr_cap) S.nbr[(size_t)cnt * cap + i] = j;
-                            ++cnt;
+                    for (int jb = j0; jb < j1; jb += 64) {
+                        const int j = jb + lane;
+                        const bool valid = j < j1;
+                        D4 pj = D4{0.0, 0.0, 0.0, 0.0};
+                        if (valid) pj = x4[j];
+                        for (int ii = 0; ii < nb; ++ii) {
+                            const int i = cb + ii;
+                            const D4 pi = x4[i];
+                            bool hit = false;
+                            if (valid && j != i) {
+                                const double maxd = pi.r + pj.r + skin;
+                                double d[3];
+                                wrapped_diff<D>(c, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, d);
+                                hit = sq_len(d) < maxd * maxd;
+                            }
+                            const unsigned long long m = __ballot(hit);
+                            if (m) {
+                                const int base = __shfl(cnt, ii);
+                                if (hit) {
+                                    const int pos = base + __popcll(m & lt);
+                                    if (pos < nbr_cap) nbr[(size_t)pos * cap + i] = j;
+                                }
+                                if (lane == ii) cnt += __popcll(m);
+                            }
                         }
                     }
                     e = f + 1;
                 }
             }
         }
-        S.nn[i] = cnt < nbr_cap ? cnt : nbr_cap;
-        sum_nn += cnt;
-        max_nn = max(max_nn, cnt);
-        if (cnt > nbr_cap) ovf = 1;
+        if (lane < nb) {
+            nn[cb + lane] = cnt < nbr_cap ? cnt : nbr_cap;
+            sum_nn += cnt;
+            max_nn = max(max_nn, cnt);
+            if (cnt > nbr_cap) ovf = 1;
+        }
     }
 }
 
+// Search grid: block b holds cells [32 b, 32 b + 32); wave w takes cells
+// 32 b + w, 32 b + w + 4, ...
 __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt) {
-    const Ctx cx = bubble_ctx(foams, bt);
+    const Ctx cx = cell_ctx(foams, bt);
     Foam *F = cx.F;
     if (!F->d.need_search) return;
     const Consts c = F->c;
-    const Roles R = F->r;
     const Scratch S = F->s;
-    const int n = F->d.count, ipt = F->ipt;
+    const int ncell = F->d.num_cells;
     const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
-    const int base = cx.lb * kBlock * ipt;
+    const int base = cx.lb * kSearchCells;
+    const int end = min(base + kSearchCells, ncell);
+    const size_t cap = (size_t)F->cap;
     int sum_nn = 0, max_nn = 0, ovf = 0;
-    if (3 == c.dim)
-        search_range<3>(c, R, S, cd, (size_t)F->cap, F->nbr_cap, base, ipt, n, sum_nn, max_nn, ovf);
-    else
-        search_range<2>(c, R, S, cd, (size_t)F->cap, F->nbr_cap, base, ipt, n, sum_nn, max_nn, ovf);
+    for (int cell = base + (int)(threadIdx.x >> 6); cell < end; cell += kWaves) {
+        if (3 == c.dim)
+            search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell, sum_nn,
+                           max_nn, ovf);
+        else
+            search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell, sum_nn,
+                           max_nn, ovf);
+    }
     sum_nn = block_sum_i(sum_nn);
     max_nn = block_max_i(max_nn);
     ovf = block_max_i(ovf);
     if (threadIdx.x == 0) {
-        S.part_i[cx.lb * kPartI + 1] = sum_nn;
-        S.part_i[cx.lb * kPartI + 2] = max_nn;
-        S.part_i[cx.lb * kPartI + 3] = ovf;
+        S.cpart_i[cx.lb * kPartI + 1] = sum_nn;
+        S.cpart_i[cx.lb * kPartI + 2] = max_nn;
+        S.cpart_i[cx.lb * kPartI + 3] = ovf;
     }
-    if (last_block(&F->d.ticket[kTkSearch], F->nblk)) {
-        const int nb = F->nblk;
-        long long s = 0;
-        int mx = 0, of = 0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) {
-            s += S.part_i[b * kPartI + 1];
-            mx = max(mx, S.part_i[b * kPartI + 2]);
-            of = max(of, S.part_i[b * kPartI + 3]);
+}
+
+// One block per foam: make the reordered buffers current (swap = 1), pair
+// count, list-capacity check, counters.
+__global__ __launch_bounds__(kBlock) void k_s_fin(Foam *foams, int swap) {
+    Foam *F = foams + blockIdx.x;
+    if (!F->d.need_search) return;
+    const Scratch S = F->s;
+    const int nb = F->sblk;
+    int s = 0, mx = 0, of = 0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        s += S.cpart_i[b * kPartI + 1];
+        mx = max(mx, S.cpart_i[b * kPartI + 2]);
+        of = max(of, S.cpart_i[b * kPartI + 3]);
+    }
+    const int tot = block_sum_i(s);
+    mx = block_max_i(mx);
+    of = block_max_i(of);
+    if (threadIdx.x == 0) {
+        if (swap) swap_reordered_roles(F->r);
+        F->d.num_pairs = tot / 2;
+        F->d.max_nn = mx;
+        F->d.searches += 1;
+        if (of) {
+            F->d.overflow = 1;
+            F->d.halt = 1;
         }
-        const int tot = block_sum_i((int)s);
-        mx = block_max_i(mx);
-        of = block_max_i(of);
-        if (threadIdx.x == 0) {
-            F->d.num_pairs = tot / 2;
-            F->d.max_nn = mx;
-            F->d.searches += 1;
-            if (of) {
-                F->d.overflow = 1;
-                F->d.halt = 1;
-            }
-            F->d.need_search = 0;
-            F->d.ticket[kTkSearch] = 0;
-        }
+        F->d.need_search = 0;
     }
 }
 
@@ -373,11 +396,15 @@ void launch_search(const Grid &g) {
     hipLaunchKernelGGL(k_s_scan2, dim3(g.nb_cell), dim3(kBlock), 0, g.stream, g.foams, g.bt_cell);
     hipLaunchKernelGGL(k_s_bin, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_s_reorder, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
-    hipLaunchKernelGGL(k_s_search, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_s_search, dim3(g.nb_scell), dim3(kBlock), 0, g.stream, g.foams,
+                       g.bt_scell);
+    hipLaunchKernelGGL(k_s_fin, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, 1);
 }
 
 void launch_search_lists(const Grid &g) {
-    hipLaunchKernelGGL(k_s_search, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    hipLaunchKernelGGL(k_s_search, dim3(g.nb_scell), dim3(kBlock), 0, g.stream, g.foams,
+                       g.bt_scell);
+    hipLaunchKernelGGL(k_s_fin, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, 0);
 }
 
 void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear_halt) {

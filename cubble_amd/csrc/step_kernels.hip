@@ -24,23 +24,25 @@ namespace cubble_amd {
 
 // ---------------------------------------------------------------- predictor
 template <int D, bool GAS>
-__device__ __forceinline__ void predict_range(const Roles &R, int base, int ipt, int n, double h,
-                                              double &sa, double &sar) {
+__device__ __forceinline__ void predict_range(const Roles &R, D4 *__restrict__ p4, int base,
+                                              int ipt, int n, double h, double &sa, double &sar) {
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
+        double p[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < D; ++q) R.xp[q][i] = R.x[q][i] + h * (3.0 * R.v[q][i] - R.vo[q][i]);
+        for (int q = 0; q < D; ++q) p[q] = R.x[q][i] + h * (3.0 * R.v[q][i] - R.vo[q][i]);
+        double rp;
         if (GAS) {
-            const double rp = R.x[3][i] + h * (3.0 * R.v[3][i] - R.vo[3][i]);
-            R.xp[3][i] = rp;
+            rp = R.x[3][i] + h * (3.0 * R.v[3][i] - R.vo[3][i]);
             double apr = 2.0 * kPi;
             if (3 == D) apr *= 2.0 * rp;
             sa += apr * rp;
             sar += apr;
         } else {
-            R.xp[3][i] = R.x[3][i];
+            rp = R.x[3][i];
         }
+        p4[i] = D4{p[0], p[1], p[2], rp};
     }
 }
 
@@ -54,12 +56,13 @@ __global__ __launch_bounds__(kBlock) void k_predict(Foam *foams, const int2 *bt,
     const int base = cx.lb * kBlock * ipt;
     double sa = 0.0, sar = 0.0;
     const bool gas = (flags & kGas) != 0;
+    D4 *p4 = F->s.p4;
     if (3 == F->c.dim) {
-        if (gas) predict_range<3, true>(R, base, ipt, n, h, sa, sar);
-        else predict_range<3, false>(R, base, ipt, n, h, sa, sar);
+        if (gas) predict_range<3, true>(R, p4, base, ipt, n, h, sa, sar);
+        else predict_range<3, false>(R, p4, base, ipt, n, h, sa, sar);
     } else {
-        if (gas) predict_range<2, true>(R, base, ipt, n, h, sa, sar);
-        else predict_range<2, false>(R, base, ipt, n, h, sa, sar);
+        if (gas) predict_range<2, true>(R, p4, base, ipt, n, h, sa, sar);
+        else predict_range<2, false>(R, p4, base, ipt, n, h, sa, sar);
     }
     sa = block_sum(sa);
     sar = block_sum(sar);
@@ -74,29 +77,28 @@ template <int D, bool GAS>
 __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, const Scratch &S,
                                            size_t cap, int base, int ipt, int n, double &soa,
                                            double &soar) {
-    const double *__restrict__ px = R.xp[0];
-    const double *__restrict__ py = R.xp[1];
-    const double *__restrict__ pz = R.xp[2];
-    const double *__restrict__ pr = R.xp[3];
+    const D4 *__restrict__ p4 = S.p4;
     const int *__restrict__ nnA = S.nn;
     const int *__restrict__ nbr = S.nbr;
     const double f0 = c.f0;
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const double xi = px[i], yi = py[i], zi = D == 3 ? pz[i] : 0.0, ri = pr[i];
+        const D4 pi = p4[i];
+        const double xi = pi.x, yi = pi.y, zi = pi.z, ri = pi.r;
         const double inv_ri = 1.0 / ri;
         double acc[3] = {0.0, 0.0, 0.0};
         double ovl = 0.0, dr = 0.0;
         const int m_nn = nnA[i];
         for (int q = 0; q < m_nn; ++q) {
             const int j = nbr[(size_t)q * cap + i];
-            const double rj = pr[j];
+            const D4 pj = p4[j];
+            const double rj = pj.r;
             const double r1 = i < j ? ri : rj;
             const double r2 = i < j ? rj : ri;
             const double radii = r1 + r2;
             double d[3];
-            wrapped_diff<D>(c, xi, yi, zi, px[j], py[j], D == 3 ? pz[j] : 0.0, d);
+            wrapped_diff<D>(c, xi, yi, zi, pj.x, pj.y, pj.z, d);
             const double m = sq_len(d);
             if (radii * radii >= m) {
                 const double s = 1.0 / sqrt(m) - 1.0 / radii;
@@ -172,19 +174,26 @@ __global__ __launch_bounds__(kBlock) void k_pair(Foam *foams, const int2 *bt, ui
         S.part[cx.lb * kPartD + 2] = soa;
         S.part[cx.lb * kPartD + 3] = soar;
     }
-    if (last_block(&F->d.ticket[kTkPair], F->nblk)) {
-        const int nb = F->nblk;
-        const double sa = reduce_parts_sum(S.part, nb, 0);
-        const double sar = reduce_parts_sum(S.part, nb, 1);
-        const double s_oa = reduce_parts_sum(S.part, nb, 2);
-        const double s_oar = reduce_parts_sum(S.part, nb, 3);
-        if (threadIdx.x == 0) {
-            F->d.s_a = sa;
-            F->d.s_ar = sar;
-            F->d.s_oa = s_oa;
-            F->d.s_oar = s_oar;
-            F->d.ticket[kTkPair] = 0;
-        }
+}
+
+// One block per foam: S_A, S_Ar, S_OA, S_OAr from the predictor and pair
+// partials, in a fixed order.  (A separate launch: on MI355X the eight XCD
+// L2 caches are not coherent, so an in-kernel last-block reduction would
+// need an agent-scope release -- an L2 write-back -- in every block.)
+__global__ __launch_bounds__(kBlock) void k_sums(Foam *foams) {
+    Foam *F = foams + blockIdx.x;
+    if (F->d.halt) return;
+    const double *part = F->s.part;
+    const int nb = F->nblk;
+    const double sa = reduce_parts_sum(part, nb, 0);
+    const double sar = reduce_parts_sum(part, nb, 1);
+    const double s_oa = reduce_parts_sum(part, nb, 2);
+    const double s_oar = reduce_parts_sum(part, nb, 3);
+    if (threadIdx.x == 0) {
+        F->d.s_a = sa;
+        F->d.s_ar = sar;
+        F->d.s_oa = s_oa;
+        F->d.s_oar = s_oar;
     }
 }
 
@@ -245,13 +254,14 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
+        const D4 pp = S.p4[i];
+        const double pv[3] = {pp.x, pp.y, pp.z};
         double vp[3];
 #pragma unroll
         for (int q = 0; q < D; ++q) vp[q] = R.vp[q][i];
         if (walls) {
             double vz = 3 == D ? vp[2] : 0.0;
-            add_wall_velocity<D>(c, R.xp[0][i], R.xp[1][i], 3 == D ? R.xp[2][i] : 0.0, R.xp[3][i],
-                                 vp[0], vp[1], vz);
+            add_wall_velocity<D>(c, pp.x, pp.y, pp.z, pp.r, vp[0], vp[1], vz);
             if (3 == D) vp[2] = vz;
 #pragma unroll
             for (int q = 0; q < D; ++q) R.vp[q][i] = vp[q];
@@ -260,7 +270,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
         double xo[3], xc[3];
 #pragma unroll
         for (int q = 0; q < D; ++q) {
-            const double pred = R.xp[q][i];
+            const double pred = pv[q];
             xo[q] = R.x[q][i];
             const double corr = xo[q] + h * (R.v[q][i] + vp[q]);
             xc[q] = corr;
@@ -272,7 +282,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
         dist = sqrt(dist);
         int dele = 0;
         if (GAS) {
-            double r = R.xp[3][i];
+            double r = pp.r;
             double area = 2.0 * kPi * r;
             if (3 == D) area *= 2.0 * r;
             double vr = inv_rho - 1.0 / r;
@@ -295,6 +305,8 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
                 a.n_del += 1;
             }
             a.max_r = fmax(a.max_r, r);
+        } else {
+            R.xp[3][i] = pp.r;  // radius unchanged without gas exchange
         }
         S.del[i] = (uint8_t)dele;
         a.max_exp = fmax(a.max_exp, dist);
@@ -407,21 +419,24 @@ __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt,
         S.part[cx.lb * kPartD + 7] = mx;
         S.part_i[cx.lb * kPartI + 0] = nd;
     }
-    if (last_block(&F->d.ticket[kTkCorr], F->nblk)) {
-        const int nb = F->nblk;
-        CorrAcc t;
-        t.s_v = reduce_parts_sum(S.part, nb, 4);
-        t.max_err = reduce_parts_max(S.part, nb, 5);
-        t.max_r = reduce_parts_max(S.part, nb, 6);
-        t.max_exp = reduce_parts_max(S.part, nb, 7);
-        int ndl = 0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) ndl += S.part_i[b * kPartI + 0];
-        t.n_del = block_sum_i(ndl);
-        if (threadIdx.x == 0) {
-            control(F, flags, t);
-            F->d.ticket[kTkCorr] = 0;
-        }
-    }
+}
+
+// One block per foam: reduce the corrector partials, then the dt controller
+// and (when committing) the role rotation.
+__global__ __launch_bounds__(kBlock) void k_control(Foam *foams, uint32_t flags) {
+    Foam *F = foams + blockIdx.x;
+    if (F->d.halt) return;
+    const Scratch S = F->s;
+    const int nb = F->nblk;
+    CorrAcc t;
+    t.s_v = reduce_parts_sum(S.part, nb, 4);
+    t.max_err = reduce_parts_max(S.part, nb, 5);
+    t.max_r = reduce_parts_max(S.part, nb, 6);
+    t.max_exp = reduce_parts_max(S.part, nb, 7);
+    int ndl = 0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) ndl += S.part_i[b * kPartI + 0];
+    t.n_del = block_sum_i(ndl);
+    if (threadIdx.x == 0) control(F, flags, t);
 }
 
 // ---------------------------------------------------------------- launches
@@ -434,9 +449,14 @@ void launch_pair(const Grid &g, uint32_t flags) {
     hipLaunchKernelGGL(k_pair, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub, flags);
 }
 
+void launch_sums(const Grid &g) {
+    hipLaunchKernelGGL(k_sums, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
+}
+
 void launch_correct(const Grid &g, uint32_t flags) {
     hipLaunchKernelGGL(k_correct, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub,
                        flags);
+    hipLaunchKernelGGL(k_control, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
 }
 
 }  // namespace cubble_amd
