@@ -107,6 +107,19 @@ class _KernelTimes(ctypes.Structure):
                 ("correct_ms", ctypes.c_double), ("reps", ctypes.c_int32)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("bubbles_per_thread", ctypes.c_int32), ("search_cells", ctypes.c_int32)]
+
+
+def _tuning(t):
+    if t is None:
+        return None
+    if isinstance(t, dict):
+        return ctypes.byref(_Tuning(int(t.get("bubbles_per_thread", 0)),
+                                    int(t.get("search_cells", 0))))
+    return ctypes.byref(_Tuning(int(t[0]), int(t[1])))
+
+
 # exported symbols of include/cubble_amd.h (checked by the ABI test)
 EXPORTS = [
     "cubble_create", "cubble_create_from_json", "cubble_config_from_json",
@@ -114,6 +127,9 @@ EXPORTS = [
     "cubble_integrate", "cubble_run_steps", "cubble_search_neighbors", "cubble_remove_bubbles",
     "cubble_get_scalars", "cubble_get_state", "cubble_get_attempt", "cubble_set_state",
     "cubble_generate_foam", "cubble_run", "cubble_version", "cubble_time_kernels",
+    "cubble_create_tuned", "cubble_batch_create", "cubble_batch_size", "cubble_batch_integrate",
+    "cubble_batch_run_steps", "cubble_batch_get_scalars", "cubble_batch_get_state",
+    "cubble_batch_last_error", "cubble_batch_destroy", "cubble_batch_time_kernels",
 ]
 
 _lib = None
@@ -150,6 +166,19 @@ def lib():
         L.cubble_version.restype = ctypes.c_char_p
         L.cubble_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
                                           P(_KernelTimes)]
+        L.cubble_create_tuned.argtypes = [P(_Config), P(_Tuning), P(vp)]
+        L.cubble_batch_create.argtypes = [P(_Config), ctypes.c_int32, P(_Tuning), P(vp)]
+        L.cubble_batch_size.argtypes = [vp]
+        L.cubble_batch_integrate.argtypes = [vp, P(_IntegrateParams), P(_StepResult)]
+        L.cubble_batch_run_steps.argtypes = [vp, P(_IntegrateParams), ctypes.c_int64,
+                                             P(_StepResult)]
+        L.cubble_batch_get_scalars.argtypes = [vp, ctypes.c_int32, P(_Scalars)]
+        L.cubble_batch_get_state.argtypes = [vp, ctypes.c_int32, P(_StateView)]
+        L.cubble_batch_last_error.argtypes = [vp]
+        L.cubble_batch_last_error.restype = ctypes.c_char_p
+        L.cubble_batch_destroy.argtypes = [vp]
+        L.cubble_batch_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
+                                                P(_KernelTimes)]
         _lib = L
     return _lib
 
@@ -267,14 +296,15 @@ class Simulator:
     """One foam on the GPU.  Mirrors the reference's step interface:
     step() (one attempt), integrate(), search_neighbors(), remove_bubbles()."""
 
-    def __init__(self, config=None, json_path=None):
+    def __init__(self, config=None, json_path=None, tuning=None):
         h = ctypes.c_void_p()
         if json_path is not None:
             _check(lib().cubble_create_from_json(json_path.encode(), ctypes.byref(h)))
         else:
             cfg = config if config is not None else Config()
             self.config = cfg
-            _check(lib().cubble_create(ctypes.byref(cfg._c()), ctypes.byref(h)))
+            _check(lib().cubble_create_tuned(ctypes.byref(cfg._c()), _tuning(tuning),
+                                             ctypes.byref(h)))
         self._h = h
 
     def close(self):
@@ -336,23 +366,7 @@ class Simulator:
 
     def get_state(self):
         sc = self.scalars()
-        n = sc["count"]
-        st = {"x": np.zeros((4, n)), "v": np.zeros((4, n)), "vo": np.zeros((4, n)),
-              "saved": np.zeros((4, n)), "path": np.zeros(n), "err": np.zeros(n),
-              "wc": np.zeros((3, n), np.int32), "idx": np.zeros(n, np.int32),
-              "nbr_off": np.zeros(n + 1, np.int32)}
-        nbr = np.zeros(max(1, sc["nbr_cap"] * n), np.int32)
-        v = _StateView()
-        for k in range(4):
-            v.x[k], v.v[k] = _dptr(st["x"][k]), _dptr(st["v"][k])
-            v.vo[k], v.saved[k] = _dptr(st["vo"][k]), _dptr(st["saved"][k])
-        for k in range(3):
-            v.wrap[k] = _iptr(st["wc"][k])
-        v.path, v.error, v.index = _dptr(st["path"]), _dptr(st["err"]), _iptr(st["idx"])
-        v.nbr_off, v.nbr, v.nbr_cap = _iptr(st["nbr_off"]), _iptr(nbr), nbr.size
-        self._call(lib().cubble_get_state(self._h, ctypes.byref(v)))
-        st["nbr"] = nbr[: st["nbr_off"][n]].copy()
-        return st, sc
+        return _read_state(sc, lambda v: self._call(lib().cubble_get_state(self._h, v))), sc
 
     def get_attempt(self):
         n = self.scalars()["count"]
@@ -409,6 +423,87 @@ class Simulator:
         """Half-list pair set (i < j), the reference's `pairs` (src/kernels.cu:703-711)."""
         st, _ = self.get_state()
         return half_pairs(st["nbr_off"], st["nbr"])
+
+
+def _read_state(sc, fill):
+    n = sc["count"]
+    st = {"x": np.zeros((4, n)), "v": np.zeros((4, n)), "vo": np.zeros((4, n)),
+          "saved": np.zeros((4, n)), "path": np.zeros(n), "err": np.zeros(n),
+          "wc": np.zeros((3, n), np.int32), "idx": np.zeros(n, np.int32),
+          "nbr_off": np.zeros(n + 1, np.int32)}
+    nbr = np.zeros(max(1, sc["nbr_cap"] * n), np.int32)
+    v = _StateView()
+    for k in range(4):
+        v.x[k], v.v[k] = _dptr(st["x"][k]), _dptr(st["v"][k])
+        v.vo[k], v.saved[k] = _dptr(st["vo"][k]), _dptr(st["saved"][k])
+    for k in range(3):
+        v.wrap[k] = _iptr(st["wc"][k])
+    v.path, v.error, v.index = _dptr(st["path"]), _dptr(st["err"]), _iptr(st["idx"])
+    v.nbr_off, v.nbr, v.nbr_cap = _iptr(st["nbr_off"]), _iptr(nbr), nbr.size
+    fill(ctypes.byref(v))
+    st["nbr"] = nbr[: st["nbr_off"][n]].copy()
+    return st
+
+
+class Batch:
+    """Many independent foams advanced together (2D and 3D mixed, uneven
+    sizes, per-foam constants); every kernel launch covers the whole batch and
+    a run of steps is a replayed HIP graph."""
+
+    def __init__(self, configs, tuning=None):
+        configs = list(configs)
+        arr = (_Config * len(configs))(*[c._c() for c in configs])
+        h = ctypes.c_void_p()
+        rc = lib().cubble_batch_create(arr, len(configs), _tuning(tuning), ctypes.byref(h))
+        if rc != OK:
+            raise CubbleError(rc, lib().cubble_batch_last_error(None).decode())
+        self._h = h
+        self.configs = configs
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().cubble_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, rc):
+        if rc != OK:
+            raise CubbleError(rc, lib().cubble_batch_last_error(self._h).decode())
+
+    def __len__(self):
+        return lib().cubble_batch_size(self._h)
+
+    def integrate(self, ip=None):
+        r = (_StepResult * len(self))()
+        self._call(lib().cubble_batch_integrate(self._h, ctypes.byref((ip or IntegrationParams())._c()), r))
+        return [_res(x) for x in r]
+
+    def run_steps(self, n, ip=None):
+        r = (_StepResult * len(self))()
+        self._call(lib().cubble_batch_run_steps(self._h, ctypes.byref((ip or IntegrationParams())._c()),
+                                                int(n), r))
+        return [_res(x) for x in r]
+
+    def scalars(self, k):
+        s = _Scalars()
+        self._call(lib().cubble_batch_get_scalars(self._h, int(k), ctypes.byref(s)))
+        return _scal(s)
+
+    def get_state(self, k):
+        sc = self.scalars(k)
+        return _read_state(sc, lambda v: self._call(lib().cubble_batch_get_state(self._h, int(k), v))), sc
+
+    def time_kernels(self, reps=20, ip=None):
+        t = _KernelTimes()
+        self._call(lib().cubble_batch_time_kernels(self._h, ctypes.byref((ip or IntegrationParams())._c()),
+                                                   int(reps), ctypes.byref(t)))
+        return {"predict_ms": t.predict_ms, "pair_ms": t.pair_ms, "correct_ms": t.correct_ms,
+                "reps": t.reps}
 
 
 def half_pairs(off, nbr):

@@ -1,0 +1,511 @@
+// engine.hip -- the internal host runtime (see engine.hpp).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#include "host_init.hpp"
+
+namespace cubble_amd {
+
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+#define E_TRY(call)                                                                         \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(e_ == hipErrorOutOfMemory ? CUBBLE_E_NOMEM : CUBBLE_E_HIP,          \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+// Assigns every per-foam buffer (except the neighbour lists) inside one
+// block starting at `base`; with base == nullptr only measures the size.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t count) {
+        const size_t o = off;
+        off = align256(off + std::max<size_t>(count, 1) * sizeof(T));
+        return base ? reinterpret_cast<T *>(base + o) : nullptr;
+    }
+};
+
+size_t carve_foam(Foam &F, char *base) {
+    Carver L{base};
+    const size_t cap = (size_t)F.cap;
+    const int D = F.c.dim;
+    double *xb[2][4] = {}, *vb[4][4] = {};
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 4; ++q)
+            if (!(2 == q && 2 == D)) xb[a][q] = L.take<double>(cap);
+    for (int a = 0; a < 4; ++a)
+        for (int q = 0; q < 4; ++q)
+            if (!(2 == q && 2 == D)) vb[a][q] = L.take<double>(cap);
+    for (int q = 0; q < 4; ++q) {
+        F.r.x[q] = xb[0][q];
+        F.r.xp[q] = xb[1][q];
+        F.r.v[q] = vb[0][q];
+        F.r.vp[q] = vb[1][q];
+        F.r.vo[q] = vb[2][q];
+        F.r.vs[q] = vb[3][q];
+        F.r.saved[q] = (2 == q && 2 == D) ? nullptr : L.take<double>(cap);
+    }
+    F.r.path = L.take<double>(cap);
+    F.r.path_new = L.take<double>(cap);
+    F.r.err = L.take<double>(cap);
+    F.r.err_alt = L.take<double>(cap);
+    for (int q = 0; q < 3; ++q) {
+        F.r.wc[q] = q < D ? L.take<int32_t>(cap) : nullptr;
+        F.r.wc_alt[q] = q < D ? L.take<int32_t>(cap) : nullptr;
+    }
+    F.r.idx = L.take<int32_t>(cap);
+    F.r.idx_alt = L.take<int32_t>(cap);
+    Scratch &S = F.s;
+    S.p4 = L.take<D4>(cap);
+    S.x4 = L.take<D4>(cap);
+    S.ov = L.take<double>(cap);
+    S.del = L.take<uint8_t>(cap);
+    S.nn = L.take<int32_t>(cap);
+    S.cell_of = L.take<int32_t>(cap);
+    S.members = L.take<int32_t>(cap);
+    S.del_list = L.take<int32_t>(cap);
+    S.plan_src = L.take<int32_t>(cap);
+    S.moved_to = L.take<int32_t>(cap);
+    S.cell_count = L.take<int32_t>((size_t)F.cell_cap);
+    S.cell_cursor = L.take<int32_t>((size_t)F.cell_cap);
+    S.cell_start = L.take<int32_t>((size_t)F.cell_cap + 1);
+    S.part = L.take<double>((size_t)F.nblk * kPartD);
+    S.part_i = L.take<int32_t>((size_t)F.nblk * kPartI);
+    S.cpart = L.take<int32_t>((size_t)F.cblk);
+    S.cpart_i = L.take<int32_t>((size_t)F.sblk * kPartI);
+    return L.off;
+}
+
+void size_foam(Foam &F, int cap, int nbr_cap, int cell_cap, const Tuning &t) {
+    F.cap = cap;
+    F.nbr_cap = nbr_cap;
+    F.cell_cap = cell_cap;
+    F.ipt = t.ipt > 0 ? t.ipt : std::max(1, (cap + kBlock * kMaxBlocks - 1) / (kBlock * kMaxBlocks));
+    F.nblk = std::max(1, (cap + kBlock * F.ipt - 1) / (kBlock * F.ipt));
+    F.cipt = std::max(1, (cell_cap + kBlock * 1024 - 1) / (kBlock * 1024));
+    F.cblk = std::max(1, (cell_cap + kBlock * F.cipt - 1) / (kBlock * F.cipt));
+    F.scells = t.scells > 0 ? t.scells : 32;
+    F.sblk = std::max(1, (cell_cap + F.scells - 1) / F.scells);
+}
+
+int default_nbr_cap(int dim) { return 3 == dim ? 48 : 24; }
+
+}  // namespace
+
+// ---------------------------------------------------------------- arena
+DeviceArena::~DeviceArena() {
+    for (auto &c : chunks_) (void)hipFree(c.base);
+}
+
+bool DeviceArena::add_chunk(size_t bytes) {
+    bytes = align256(bytes);
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return false;
+    Chunk c;
+    c.base = static_cast<char *>(p);
+    c.size = bytes;
+    c.free[0] = bytes;
+    chunks_.push_back(std::move(c));
+    capacity_ += bytes;
+    return true;
+}
+
+int DeviceArena::init(size_t first_chunk, bool growable, std::string &err) {
+    growable_ = growable;
+    if (first_chunk > 0 && !add_chunk(first_chunk)) {
+        err = "hipMalloc of the device arena failed";
+        return CUBBLE_E_NOMEM;
+    }
+    return CUBBLE_OK;
+}
+
+void *DeviceArena::alloc(size_t bytes) {
+    bytes = align256(std::max<size_t>(bytes, 256));
+    for (int pass = 0; pass < 2; ++pass) {
+        for (auto &c : chunks_) {
+            for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+                if (it->second < bytes) continue;
+                const size_t off = it->first, len = it->second;
+                c.free.erase(it);
+                if (len > bytes) c.free[off + bytes] = len - bytes;
+                void *p = c.base + off;
+                live_[p] = bytes;
+                in_use_ += bytes;
+                return p;
+            }
+        }
+        if (pass == 0) {
+            if (!growable_) return nullptr;
+            if (!add_chunk(std::max(bytes, (size_t)64 << 20))) return nullptr;
+        }
+    }
+    return nullptr;
+}
+
+void DeviceArena::release(void *p) {
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    size_t len = it->second;
+    live_.erase(it);
+    in_use_ -= len;
+    for (auto &c : chunks_) {
+        char *q = static_cast<char *>(p);
+        if (q < c.base || q >= c.base + c.size) continue;
+        size_t off = (size_t)(q - c.base);
+        auto nx = c.free.lower_bound(off);
+        if (nx != c.free.end() && nx->first == off + len) {
+            len += nx->second;
+            nx = c.free.erase(nx);
+        }
+        if (nx != c.free.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {
+                pv->second += len;
+                return;
+            }
+        }
+        c.free[off] = len;
+        return;
+    }
+}
+
+size_t DeviceArena::largest_free() const {
+    size_t m = 0;
+    for (const auto &c : chunks_)
+        for (const auto &f : c.free) m = std::max(m, f.second);
+    return m;
+}
+
+size_t foam_footprint(int dim, int cap, int nbr_cap, int cell_cap, const Tuning &t) {
+    Foam F;
+    std::memset(&F, 0, sizeof(F));
+    F.c.dim = dim;
+    size_foam(F, cap, nbr_cap > 0 ? nbr_cap : default_nbr_cap(dim), cell_cap, t);
+    return carve_foam(F, nullptr) + align256((size_t)F.nbr_cap * cap * sizeof(int32_t));
+}
+
+// ---------------------------------------------------------------- engine
+Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (void *p : {(void *)dfoams_, (void *)bt_bub_, (void *)bt_cell_, (void *)bt_scell_})
+        if (p) (void)hipFree(p);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Engine::init(size_t arena_bytes, bool growable) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(CUBBLE_E_HIP, "no HIP device found");
+    if (hipGetDevice(&device) != hipSuccess) device = 0;
+    E_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    std::string e;
+    const int rc = arena.init(arena_bytes, growable, e);
+    if (rc) return fail(rc, e);
+    return CUBBLE_OK;
+}
+
+int Engine::add_foam(const cubble_config &cfg, const Tuning &t, int *index) {
+    std::vector<double> x4;
+    InitOut io;
+    std::string e;
+    int rc = generate_foam(cfg, x4, io, e);
+    if (rc) return fail(rc, e);
+    if (!foams.empty() && cfg.precision != precision)
+        return fail(CUBBLE_E_INVALID, "all foams of one batch must share the precision mode");
+    precision = cfg.precision;
+    const int n = cfg.num_start;
+    FoamRec r;
+    Foam &F = r.h;
+    std::memset(&F, 0, sizeof(F));
+    F.c = io.c;
+    F.precision = cfg.precision;
+    size_foam(F, n, cfg.nbr_cap > 0 ? cfg.nbr_cap : default_nbr_cap(cfg.dim), io.cell_cap, t);
+    const size_t bytes = carve_foam(F, nullptr);
+    r.block = arena.alloc(bytes);
+    r.nbr_block = r.block ? arena.alloc((size_t)F.nbr_cap * n * sizeof(int32_t)) : nullptr;
+    if (!r.block || !r.nbr_block) {
+        if (r.block) arena.release(r.block);
+        return fail(CUBBLE_E_NOMEM, "device memory for the foam is not available");
+    }
+    carve_foam(F, static_cast<char *>(r.block));
+    F.s.nbr = static_cast<int32_t *>(r.nbr_block);
+    E_TRY(hipMemsetAsync(r.block, 0, bytes, stream));
+    for (int q = 0; q < 4; ++q) {
+        if (2 == q && 2 == cfg.dim) continue;
+        E_TRY(hipMemcpyAsync(F.r.x[q], x4.data() + (size_t)q * n, sizeof(double) * n,
+                             hipMemcpyHostToDevice, stream));
+    }
+    std::vector<int32_t> iota(n);
+    for (int i = 0; i < n; ++i) iota[i] = i;
+    E_TRY(hipMemcpyAsync(F.r.idx, iota.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+    E_TRY(hipStreamSynchronize(stream));
+    Dyn &d = F.d;
+    d.count = n;
+    d.dt = cfg.dt0;
+    d.max_r_cell = io.max_r;
+    d.need_search = 1;
+    r.num_end = cfg.num_end;
+    r.max_steps = cfg.max_steps;
+    foams.push_back(r);
+    dev_index_.push_back(-1);
+    if (index) *index = (int)foams.size() - 1;
+    return CUBBLE_OK;
+}
+
+int Engine::drop_foam(int k) {
+    if (k < 0 || k >= (int)foams.size()) return fail(CUBBLE_E_INVALID, "no such foam");
+    // keep the device state of the remaining foams before the array changes
+    if (dfoams_) {
+        E_TRY(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < foams.size(); ++i)
+            if (dev_index_[i] >= 0)
+                E_TRY(hipMemcpy(&foams[i].h, dfoams_ + dev_index_[i], sizeof(Foam),
+                                hipMemcpyDeviceToHost));
+        for (auto &di : dev_index_) di = -2;  // host mirrors are current
+    }
+    arena.release(foams[k].block);
+    arena.release(foams[k].nbr_block);
+    foams.erase(foams.begin() + k);
+    dev_index_.erase(dev_index_.begin() + k);
+    return CUBBLE_OK;
+}
+
+int Engine::commit_layout() {
+    E_TRY(hipStreamSynchronize(stream));
+    const int nf = (int)foams.size();
+    // device state of foams already on the device is the truth
+    for (int i = 0; i < nf; ++i)
+        if (dev_index_[i] >= 0 && dfoams_)
+            E_TRY(hipMemcpy(&foams[i].h, dfoams_ + dev_index_[i], sizeof(Foam),
+                            hipMemcpyDeviceToHost));
+    if (nf > dfoams_cap_) {
+        if (dfoams_) (void)hipFree(dfoams_);
+        dfoams_ = nullptr;
+        const int cap = std::max(nf, 2 * dfoams_cap_);
+        E_TRY(hipMalloc(&dfoams_, sizeof(Foam) * std::max(cap, 1)));
+        dfoams_cap_ = cap;
+        if (pinned_) (void)hipHostFree(pinned_);
+        pinned_ = nullptr;
+        E_TRY(hipHostMalloc((void **)&pinned_, sizeof(Dyn) * std::max(cap, 1), hipHostMallocDefault));
+        pinned_cap_ = cap;
+    }
+    std::vector<Foam> hs(std::max(nf, 1));
+    std::vector<int2> bb, bc, bs;
+    for (int f = 0; f < nf; ++f) {
+        hs[f] = foams[f].h;
+        for (int l = 0; l < hs[f].nblk; ++l) bb.push_back(make_int2(f, l));
+        for (int l = 0; l < hs[f].cblk; ++l) bc.push_back(make_int2(f, l));
+        for (int l = 0; l < hs[f].sblk; ++l) bs.push_back(make_int2(f, l));
+        dev_index_[f] = f;
+    }
+    if (nf > 0) E_TRY(hipMemcpy(dfoams_, hs.data(), sizeof(Foam) * nf, hipMemcpyHostToDevice));
+    auto upload = [&](int2 *&buf, size_t &cap, const std::vector<int2> &v) -> int {
+        if (v.size() > cap) {
+            if (buf) (void)hipFree(buf);
+            buf = nullptr;
+            cap = std::max(v.size(), 2 * cap);
+            E_TRY(hipMalloc(&buf, sizeof(int2) * cap));
+        }
+        if (!v.empty()) E_TRY(hipMemcpy(buf, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice));
+        return CUBBLE_OK;
+    };
+    int rc;
+    if ((rc = upload(bt_bub_, bt_bub_cap_, bb)) || (rc = upload(bt_cell_, bt_cell_cap_, bc)) ||
+        (rc = upload(bt_scell_, bt_scell_cap_, bs)))
+        return rc;
+    nb_bub_ = (int)bb.size();
+    nb_cell_ = (int)bc.size();
+    nb_scell_ = (int)bs.size();
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    return CUBBLE_OK;
+}
+
+Grid Engine::grid() const {
+    Grid g;
+    g.foams = dfoams_;
+    g.bt_bub = bt_bub_;
+    g.nb_bub = nb_bub_;
+    g.bt_cell = bt_cell_;
+    g.nb_cell = nb_cell_;
+    g.bt_scell = bt_scell_;
+    g.nb_scell = nb_scell_;
+    g.nfoams = (int)foams.size();
+    g.stream = stream;
+    return g;
+}
+
+void Engine::enqueue_attempt(uint32_t flags) {
+    const Grid g = grid();
+    launch_predict(g, flags);
+    if (CUBBLE_PREC_MIXED == precision) launch_pair_mixed(g, flags);
+    else launch_pair(g, flags);
+    launch_sums(g);
+    launch_correct(g, flags);
+}
+
+void Engine::enqueue_group(uint32_t flags) {
+    const Grid g = grid();
+    enqueue_attempt(flags);
+    launch_remove(g);
+    launch_search(g);
+}
+
+int Engine::capture(uint32_t flags, int groups) {
+    if (graph_ && graph_flags_ == flags && graph_groups_ == groups) return CUBBLE_OK;
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    E_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < groups; ++k) enqueue_group(flags);
+    E_TRY(hipStreamEndCapture(stream, &gr));
+    const hipError_t e = hipGraphInstantiate(&graph_, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+        graph_ = nullptr;
+        return fail(CUBBLE_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    graph_flags_ = flags;
+    graph_groups_ = groups;
+    return CUBBLE_OK;
+}
+
+int Engine::sync() {
+    E_TRY(hipStreamSynchronize(stream));
+    return CUBBLE_OK;
+}
+
+int Engine::read_dyn_all(std::vector<Dyn> &out) {
+    const int nf = (int)foams.size();
+    out.resize(nf);
+    if (nf == 0) return CUBBLE_OK;
+    E_TRY(hipMemcpy2DAsync(pinned_, sizeof(Dyn), &dfoams_->d, sizeof(Foam), sizeof(Dyn), nf,
+                           hipMemcpyDeviceToHost, stream));
+    E_TRY(hipStreamSynchronize(stream));
+    std::memcpy(out.data(), pinned_, sizeof(Dyn) * nf);
+    return CUBBLE_OK;
+}
+
+int Engine::write_dyn(int k, const Dyn &d) {
+    E_TRY(hipStreamSynchronize(stream));
+    E_TRY(hipMemcpy(&dfoams_[k].d, &d, sizeof(Dyn), hipMemcpyHostToDevice));
+    return CUBBLE_OK;
+}
+
+int Engine::read_header(int k, Foam &f) {
+    E_TRY(hipStreamSynchronize(stream));
+    E_TRY(hipMemcpy(&f, dfoams_ + k, sizeof(Foam), hipMemcpyDeviceToHost));
+    return CUBBLE_OK;
+}
+
+int Engine::grow_lists(int k, int need, bool rebuild) {
+    Foam f;
+    int rc = read_header(k, f);
+    if (rc) return rc;
+    FoamRec &r = foams[k];
+    const int ncap = std::max(need + need / 4 + 4, f.nbr_cap * 2);
+    void *nb = arena.alloc((size_t)ncap * f.cap * sizeof(int32_t));
+    if (!nb) return fail(CUBBLE_E_NOMEM, "no device memory to grow the neighbour lists");
+    arena.release(r.nbr_block);
+    r.nbr_block = nb;
+    int32_t *p = static_cast<int32_t *>(nb);
+    r.h.s.nbr = p;
+    r.h.nbr_cap = ncap;
+    E_TRY(hipMemcpy(&dfoams_[k].s.nbr, &p, sizeof(p), hipMemcpyHostToDevice));
+    E_TRY(hipMemcpy(&dfoams_[k].nbr_cap, &ncap, sizeof(ncap), hipMemcpyHostToDevice));
+    if (!rebuild) return CUBBLE_OK;
+    Dyn d = f.d;
+    d.need_search = 1;
+    d.halt = 0;
+    d.overflow = 0;
+    if ((rc = write_dyn(k, d))) return rc;
+    launch_search_lists(grid());  // only foam k has its search flag set
+    E_TRY(hipGetLastError());
+    if ((rc = read_header(k, f))) return rc;
+    if (f.d.overflow) return fail(CUBBLE_E_CAPACITY, "neighbour list capacity growth failed");
+    f.d.searches -= 1;  // the rebuild is not a new search
+    return write_dyn(k, f.d);
+}
+
+int Engine::settle(std::vector<Dyn> &d) {
+    int rc = read_dyn_all(d);
+    if (rc) return rc;
+    bool grew = false;
+    for (int k = 0; k < (int)d.size(); ++k) {
+        if (!d[k].overflow) continue;
+        // the step was committed; other foams of the batch are unaffected
+        const int64_t limit = d[k].step_limit;
+        if ((rc = grow_lists(k, d[k].max_nn, true))) return rc;
+        Foam f;
+        if ((rc = read_header(k, f))) return rc;
+        f.d.halt = (limit > 0 && f.d.steps >= limit) ? 1 : 0;
+        if ((rc = write_dyn(k, f.d))) return rc;
+        grew = true;
+    }
+    if (grew) rc = read_dyn_all(d);
+    return rc;
+}
+
+int Engine::integrate(uint32_t flags, std::vector<Dyn> &d) {
+    const Grid g = grid();
+    launch_set_limits(g, 1);
+    E_TRY(hipGetLastError());
+    int rc = CUBBLE_OK;
+    for (int tries = 0; tries < 100000; ++tries) {
+        enqueue_group(flags);
+        E_TRY(hipGetLastError());
+        if ((rc = settle(d))) return rc;
+        bool done = true;
+        for (const Dyn &x : d) done = done && x.steps >= x.step_limit;
+        if (done) {
+            launch_set_limits(g, 0);
+            return read_dyn_all(d);
+        }
+    }
+    return fail(CUBBLE_E_STATE, "integrate: no accepted step after 100000 attempts");
+}
+
+int Engine::run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d) {
+    const Grid g = grid();
+    if (steps <= 0) return read_dyn_all(d);
+    const int groups = 16;
+    int rc = capture(flags, groups);
+    if (rc) return rc;
+    launch_set_limits(g, steps);
+    E_TRY(hipGetLastError());
+    for (int64_t guard = 0; guard < (int64_t)1 << 40; ++guard) {
+        if ((rc = read_dyn_all(d))) return rc;
+        int64_t left = 0;
+        for (const Dyn &x : d) left = std::max<int64_t>(left, x.step_limit - x.steps);
+        if (left <= 0) break;
+        const int64_t launches = std::max<int64_t>(1, (left + groups - 1) / groups);
+        for (int64_t k = 0; k < launches; ++k) E_TRY(hipGraphLaunch(graph_, stream));
+        if ((rc = settle(d))) return rc;
+    }
+    launch_set_limits(g, 0);
+    E_TRY(hipGetLastError());
+    return read_dyn_all(d);
+}
+
+int Engine::search_pending() {
+    launch_search(grid());
+    E_TRY(hipGetLastError());
+    std::vector<Dyn> d;
+    return settle(d);
+}
+
+}  // namespace cubble_amd

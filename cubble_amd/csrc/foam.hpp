@@ -111,11 +111,10 @@ struct Foam {
     int32_t ipt;         // bubbles per thread in bubble-space kernels
     int32_t cblk;        // cell-space blocks
     int32_t cipt;        // cells per thread in cell-space kernels
-    int32_t sblk;        // neighbour-search blocks (kSearchCells cells each)
+    int32_t sblk;        // neighbour-search blocks
+    int32_t scells;      // cells per neighbour-search block
     int32_t precision;
 };
-
-constexpr int kSearchCells = 32;  // cells per neighbour-search block (8 per wave)
 
 // launch flags
 enum Flags : uint32_t {

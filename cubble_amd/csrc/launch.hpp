@@ -15,7 +15,7 @@ struct Grid {
     int nb_bub;
     const int2 *bt_cell;
     int nb_cell;
-    const int2 *bt_scell;  // neighbour-search grid (kSearchCells cells per block)
+    const int2 *bt_scell;  // neighbour-search grid (Foam::scells cells per block)
     int nb_scell;
     int nfoams;
     hipStream_t stream;
@@ -43,5 +43,7 @@ void launch_search_lists(const Grid &g);
 
 // Set / clear the phase flags of every foam in the launch.
 void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear_halt);
+// add > 0: every foam halts after `add` more accepted steps; 0: no limit.
+void launch_set_limits(const Grid &g, long long add);
 
 }  // namespace cubble_amd

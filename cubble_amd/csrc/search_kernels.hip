@@ -318,8 +318,8 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     }
 }
 
-// Search grid: block b holds cells [32 b, 32 b + 32); wave w takes cells
-// 32 b + w, 32 b + w + 4, ...
+// Search grid: block b holds cells [s b, s b + s), s = scells; wave w takes
+// cells s b + w, s b + w + 4, ...
 __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt) {
     const Ctx cx = cell_ctx(foams, bt);
     Foam *F = cx.F;
@@ -328,8 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
     const Scratch S = F->s;
     const int ncell = F->d.num_cells;
     const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
-    const int base = cx.lb * kSearchCells;
-    const int end = min(base + kSearchCells, ncell);
+    const int base = cx.lb * F->scells;
+    const int end = min(base + F->scells, ncell);
     const size_t cap = (size_t)F->cap;
     int sum_nn = 0, max_nn = 0, ovf = 0;
     for (int cell = base + (int)(threadIdx.x >> 6); cell < end; cell += kWaves) {
@@ -388,6 +388,22 @@ __global__ void k_set_flags(Foam *foams, int nf, int ns, int nr, int clear_halt)
         foams[f].d.halt = 0;
         foams[f].d.overflow = 0;
     }
+}
+
+// add > 0: every foam halts after `add` more accepted steps; add == 0:
+// no limit.  Clears halt / overflow.
+__global__ void k_set_limits(Foam *foams, int nf, long long add) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    Dyn &d = foams[f].d;
+    d.step_limit = add > 0 ? d.steps + add : 0;
+    d.halt = 0;
+    d.overflow = 0;
+}
+
+void launch_set_limits(const Grid &g, long long add) {
+    const int nb = (g.nfoams + 63) / 64;
+    hipLaunchKernelGGL(k_set_limits, dim3(nb), dim3(64), 0, g.stream, g.foams, g.nfoams, add);
 }
 
 void launch_search(const Grid &g) {

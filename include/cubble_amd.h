@@ -201,6 +201,45 @@ int cubble_get_attempt(cubble_sim *sim, cubble_attempt_view *view);
 /* Replaces the whole state (count, arrays, scalars dt/time/max_r_cell/counters). */
 int cubble_set_state(cubble_sim *sim, const cubble_state_view *view, const cubble_scalars *sc);
 
+/* ---- launch shapes ------------------------------------------------------- */
+
+/* Launch-shape parameters picked by the autotuner (0 = built-in default).
+ * No counterpart in the reference (its launches are fixed at 5120 x 384,
+ * src/data_definitions.h:28-29). */
+typedef struct cubble_tuning {
+    int32_t bubbles_per_thread;  /* bubble-space kernels                     */
+    int32_t search_cells;        /* cells per neighbour-search block          */
+} cubble_tuning;
+
+/* cubble_create with explicit launch shapes. */
+int cubble_create_tuned(const cubble_config *cfg, const cubble_tuning *tuning, cubble_sim **out);
+
+/* ---- batched foams --------------------------------------------------------- */
+
+/* Many independent foams (2D and 3D mixed, uneven sizes, each with its own
+ * constants) advanced together: every kernel of a step is launched once for
+ * the whole batch over a block table, the accept / reject, dt, removal and
+ * search decisions are taken per foam on the device, and a run of steps is
+ * a captured HIP graph replayed without host synchronisation per step.
+ * No counterpart in the reference (one foam per process).  Each foam's
+ * results are bit-identical to the same foam run alone. */
+typedef struct cubble_batch cubble_batch;
+
+/* All configurations must share `precision`. */
+int cubble_batch_create(const cubble_config *cfgs, int32_t n, const cubble_tuning *tuning,
+                        cubble_batch **out);
+int cubble_batch_size(const cubble_batch *b);
+/* One accepted integrate() step for every foam. per_foam: n results or NULL. */
+int cubble_batch_integrate(cubble_batch *b, const cubble_integrate_params *ip,
+                           cubble_step_result *per_foam);
+/* `num_steps` accepted steps for every foam, device-driven. */
+int cubble_batch_run_steps(cubble_batch *b, const cubble_integrate_params *ip, int64_t num_steps,
+                           cubble_step_result *per_foam);
+int cubble_batch_get_scalars(cubble_batch *b, int32_t k, cubble_scalars *out);
+int cubble_batch_get_state(cubble_batch *b, int32_t k, cubble_state_view *view);
+const char *cubble_batch_last_error(const cubble_batch *b);
+void cubble_batch_destroy(cubble_batch *b);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* Average device time of each kernel of one attempt, measured with HIP
@@ -216,6 +255,8 @@ typedef struct cubble_kernel_times {
 } cubble_kernel_times;
 int cubble_time_kernels(cubble_sim *sim, const cubble_integrate_params *ip, int32_t reps,
                         cubble_kernel_times *out);
+int cubble_batch_time_kernels(cubble_batch *b, const cubble_integrate_params *ip, int32_t reps,
+                              cubble_kernel_times *out);
 
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 
