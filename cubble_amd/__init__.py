@@ -130,7 +130,32 @@ EXPORTS = [
     "cubble_create_tuned", "cubble_batch_create", "cubble_batch_size", "cubble_batch_integrate",
     "cubble_batch_run_steps", "cubble_batch_get_scalars", "cubble_batch_get_state",
     "cubble_batch_last_error", "cubble_batch_destroy", "cubble_batch_time_kernels",
+    "cubble_server_create", "cubble_server_submit", "cubble_server_run", "cubble_server_result",
+    "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
+    "cubble_foam_footprint",
 ]
+
+
+class _ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_active", ctypes.c_int32),
+                ("round_steps", ctypes.c_int32), ("keep_states", ctypes.c_int32),
+                ("precision", ctypes.c_int32)]
+
+
+class _RequestResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int32),
+                ("start_count", ctypes.c_int32), ("final_count", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("steps", ctypes.c_int64),
+                ("bubble_steps", ctypes.c_double), ("arrival_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("rounds", ctypes.c_int64),
+                ("max_resident", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("wall_s", ctypes.c_double), ("bubble_steps", ctypes.c_double),
+                ("throughput", ctypes.c_double), ("p50_latency_s", ctypes.c_double),
+                ("p99_latency_s", ctypes.c_double), ("peak_bytes", ctypes.c_double)]
 
 _lib = None
 
@@ -179,6 +204,17 @@ def lib():
         L.cubble_batch_destroy.argtypes = [vp]
         L.cubble_batch_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
                                                 P(_KernelTimes)]
+        L.cubble_server_create.argtypes = [P(_ServerOpts), P(vp)]
+        L.cubble_server_submit.argtypes = [vp, P(_Config), ctypes.c_int64, ctypes.c_double,
+                                           ctypes.c_double, P(ctypes.c_int64)]
+        L.cubble_server_run.argtypes = [vp, P(_ServerStats)]
+        L.cubble_server_result.argtypes = [vp, ctypes.c_int64, P(_RequestResult)]
+        L.cubble_server_get_state.argtypes = [vp, ctypes.c_int64, P(_StateView), P(_Scalars)]
+        L.cubble_server_last_error.argtypes = [vp]
+        L.cubble_server_last_error.restype = ctypes.c_char_p
+        L.cubble_server_destroy.argtypes = [vp]
+        L.cubble_foam_footprint.argtypes = [P(_Config)]
+        L.cubble_foam_footprint.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -504,6 +540,66 @@ class Batch:
                                                    int(reps), ctypes.byref(t)))
         return {"predict_ms": t.predict_ms, "pair_ms": t.pair_ms, "correct_ms": t.correct_ms,
                 "reps": t.reps}
+
+
+class Server:
+    """Request loop with continuous batching: requests are admitted in arrival
+    order when their device footprint fits `hbm_budget_bytes`, stay resident on
+    the GPU and advance together in rounds of `round_steps` steps."""
+
+    def __init__(self, hbm_budget_bytes, max_active=0, round_steps=50, keep_states=False,
+                 precision=PREC_FP64):
+        o = _ServerOpts(int(hbm_budget_bytes), int(max_active), int(round_steps),
+                        int(bool(keep_states)), int(precision))
+        h = ctypes.c_void_p()
+        rc = lib().cubble_server_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != OK:
+            raise CubbleError(rc, "cubble_server_create failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().cubble_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, rc):
+        if rc != OK:
+            raise CubbleError(rc, lib().cubble_server_last_error(self._h).decode())
+
+    def submit(self, config, max_steps, dissolve_fraction=0.0, arrival_s=0.0):
+        rid = ctypes.c_int64()
+        self._call(lib().cubble_server_submit(self._h, ctypes.byref(config._c()), int(max_steps),
+                                              float(dissolve_fraction), float(arrival_s),
+                                              ctypes.byref(rid)))
+        return rid.value
+
+    def run(self):
+        st = _ServerStats()
+        self._call(lib().cubble_server_run(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in _ServerStats._fields_ if name != "pad"}
+
+    def result(self, rid):
+        r = _RequestResult()
+        self._call(lib().cubble_server_result(self._h, int(rid), ctypes.byref(r)))
+        return {name: getattr(r, name) for name, _ in _RequestResult._fields_ if name != "pad"}
+
+    def get_state(self, rid):
+        s = _Scalars()
+        self._call(lib().cubble_server_get_state(self._h, int(rid), None, ctypes.byref(s)))
+        sc = _scal(s)
+        return _read_state(sc, lambda v: self._call(
+            lib().cubble_server_get_state(self._h, int(rid), v, None))), sc
+
+
+def foam_footprint(config):
+    """Device bytes one foam of this configuration occupies (admission size)."""
+    return lib().cubble_foam_footprint(ctypes.byref(config._c()))
 
 
 def half_pairs(off, nbr):

@@ -218,8 +218,13 @@ int Engine::add_foam(const cubble_config &cfg, const Tuning &t, int *index) {
     std::vector<double> x4;
     InitOut io;
     std::string e;
-    int rc = generate_foam(cfg, x4, io, e);
+    const int rc = generate_foam(cfg, x4, io, e);
     if (rc) return fail(rc, e);
+    return add_generated(cfg, t, x4, io, index);
+}
+
+int Engine::add_generated(const cubble_config &cfg, const Tuning &t, const std::vector<double> &x4,
+                          const InitOut &io, int *index) {
     if (!foams.empty() && cfg.precision != precision)
         return fail(CUBBLE_E_INVALID, "all foams of one batch must share the precision mode");
     precision = cfg.precision;
@@ -452,7 +457,10 @@ int Engine::settle(std::vector<Dyn> &d) {
         if ((rc = grow_lists(k, d[k].max_nn, true))) return rc;
         Foam f;
         if ((rc = read_header(k, f))) return rc;
-        f.d.halt = (limit > 0 && f.d.steps >= limit) ? 1 : 0;
+        f.d.halt = ((limit > 0 && f.d.steps >= limit) ||
+                    (f.d.count_stop > 0 && f.d.count <= f.d.count_stop))
+                       ? 1
+                       : 0;
         if ((rc = write_dyn(k, f.d))) return rc;
         grew = true;
     }
@@ -460,44 +468,63 @@ int Engine::settle(std::vector<Dyn> &d) {
     return rc;
 }
 
-int Engine::integrate(uint32_t flags, std::vector<Dyn> &d) {
-    const Grid g = grid();
-    launch_set_limits(g, 1);
+namespace {
+// A foam is finished for the current run once the device halted it (step
+// limit reached or dissolution stop) and no capacity rebuild is pending;
+// returns the most steps any unfinished foam still has to go.
+int64_t steps_left(const std::vector<Dyn> &d) {
+    int64_t left = 0;
+    for (const Dyn &x : d)
+        if (!(x.halt && !x.overflow)) left = std::max<int64_t>(left, std::max<int64_t>(1, x.step_limit - x.steps));
+    return left;
+}
+}  // namespace
+
+int Engine::set_limits(int64_t steps, const std::vector<int64_t> *per_foam) {
+    launch_set_limits(grid(), steps);
     E_TRY(hipGetLastError());
-    int rc = CUBBLE_OK;
+    if (!per_foam) return CUBBLE_OK;
+    std::vector<Dyn> d;
+    int rc = read_dyn_all(d);
+    if (rc) return rc;
+    for (size_t k = 0; k < d.size() && k < per_foam->size(); ++k) {
+        const int64_t lim = d[k].steps + std::max<int64_t>(1, (*per_foam)[k]);
+        E_TRY(hipMemcpy(&dfoams_[k].d.step_limit, &lim, sizeof(lim), hipMemcpyHostToDevice));
+    }
+    return CUBBLE_OK;
+}
+
+int Engine::integrate(uint32_t flags, std::vector<Dyn> &d) {
+    int rc = set_limits(1, nullptr);
+    if (rc) return rc;
     for (int tries = 0; tries < 100000; ++tries) {
         enqueue_group(flags);
         E_TRY(hipGetLastError());
         if ((rc = settle(d))) return rc;
-        bool done = true;
-        for (const Dyn &x : d) done = done && x.steps >= x.step_limit;
-        if (done) {
-            launch_set_limits(g, 0);
+        if (steps_left(d) == 0) {
+            if ((rc = set_limits(0, nullptr))) return rc;
             return read_dyn_all(d);
         }
     }
     return fail(CUBBLE_E_STATE, "integrate: no accepted step after 100000 attempts");
 }
 
-int Engine::run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d) {
-    const Grid g = grid();
+int Engine::run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d,
+                      const std::vector<int64_t> *per_foam) {
     if (steps <= 0) return read_dyn_all(d);
     const int groups = 16;
     int rc = capture(flags, groups);
     if (rc) return rc;
-    launch_set_limits(g, steps);
-    E_TRY(hipGetLastError());
+    if ((rc = set_limits(steps, per_foam))) return rc;
     for (int64_t guard = 0; guard < (int64_t)1 << 40; ++guard) {
         if ((rc = read_dyn_all(d))) return rc;
-        int64_t left = 0;
-        for (const Dyn &x : d) left = std::max<int64_t>(left, x.step_limit - x.steps);
+        const int64_t left = steps_left(d);
         if (left <= 0) break;
         const int64_t launches = std::max<int64_t>(1, (left + groups - 1) / groups);
         for (int64_t k = 0; k < launches; ++k) E_TRY(hipGraphLaunch(graph_, stream));
         if ((rc = settle(d))) return rc;
     }
-    launch_set_limits(g, 0);
-    E_TRY(hipGetLastError());
+    if ((rc = set_limits(0, nullptr))) return rc;
     return read_dyn_all(d);
 }
 

@@ -13,6 +13,7 @@
 
 #include "cubble_amd.h"
 #include "foam.hpp"
+#include "host_init.hpp"
 #include "launch.hpp"
 
 namespace cubble_amd {
@@ -60,12 +61,23 @@ struct FoamRec {
 // Bytes of device memory one foam needs (the admission footprint).
 size_t foam_footprint(int dim, int cap, int nbr_cap, int cell_cap, const Tuning &t);
 
+class Engine;
+// Copies of foam k's state between the device and caller-owned host views
+// (state_io.hip).
+int get_scalars(Engine &e, int k, cubble_scalars *out);
+int get_state(Engine &e, int k, cubble_state_view *view);
+int get_attempt(Engine &e, int k, cubble_attempt_view *view);
+int set_state(Engine &e, int k, const cubble_state_view *view, const cubble_scalars *sc);
+
 class Engine {
   public:
     ~Engine();
     int init(size_t arena_bytes, bool growable);
     // Draw the synthetic foam, allocate and upload it (no search yet).
     int add_foam(const cubble_config &cfg, const Tuning &t, int *index);
+    // Allocate and upload a foam drawn earlier by generate_foam.
+    int add_generated(const cubble_config &cfg, const Tuning &t, const std::vector<double> &x4,
+                      const InitOut &io, int *index);
     // Free a foam's memory and drop it (layout must be committed again).
     int drop_foam(int index);
     // Rebuild block tables, upload every header, drop the captured graph.
@@ -89,9 +101,13 @@ class Engine {
     // Host-driven: one attempt group at a time until each foam accepted one
     // step (integrate()).
     int integrate(uint32_t flags, std::vector<Dyn> &d);
-    // Device-driven: `steps` accepted steps for every foam through graph
-    // replays (foams that reach their limit halt on the device).
-    int run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d);
+    // Device-driven: `steps` accepted steps for every foam (or per_foam[k]
+    // for foam k) through graph replays; foams that reach their limit or
+    // their dissolution stop halt on the device.
+    int run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d,
+                  const std::vector<int64_t> *per_foam = nullptr);
+    // Step limits for the next run (0 = none); clears halt and overflow.
+    int set_limits(int64_t steps, const std::vector<int64_t> *per_foam);
 
     int fail(int code, const std::string &m) {
         err = m;

@@ -87,7 +87,7 @@ struct Dyn {
     int32_t n_holes;
     int32_t num_pairs;
     int32_t max_nn;
-    int32_t pad0;
+    int32_t count_stop;  // halt once count <= count_stop after a removal (0 = off)
     double dt;
     uint64_t time_int;
     double time_frac;

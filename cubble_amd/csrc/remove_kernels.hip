@@ -157,6 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_rm_fin(Foam *foams) {
         F->d.removals += 1;
         F->d.num_pairs = s / 2;
         F->d.need_remove = 0;
+        if (F->d.count_stop > 0 && F->d.count <= F->d.count_stop) F->d.halt = 1;
     }
 }
 

@@ -240,6 +240,68 @@ int cubble_batch_get_state(cubble_batch *b, int32_t k, cubble_state_view *view);
 const char *cubble_batch_last_error(const cubble_batch *b);
 void cubble_batch_destroy(cubble_batch *b);
 
+/* ---- serving ----------------------------------------------------------------- */
+
+/* A request loop for foam simulations with continuous batching: requests are
+ * admitted in arrival order when their device-memory footprint fits the
+ * budget (and a resident slot is free), their state stays resident on the
+ * GPU, and all resident foams advance together in rounds of `round_steps`
+ * accepted steps; requests that arrive while others run are admitted at the
+ * next round boundary, and finished requests free their memory.  A request
+ * finishes after `max_steps` accepted steps or once its bubble count has
+ * fallen to (1 - dissolve_fraction) of the start.  No counterpart in the
+ * reference.  Each request's result equals the same foam run alone. */
+typedef struct cubble_server cubble_server;
+
+typedef struct cubble_server_opts {
+    uint64_t hbm_budget_bytes; /* device memory for resident foams             */
+    int32_t max_active;        /* resident foams at once (0 = no limit)         */
+    int32_t round_steps;       /* accepted steps per round (0 = 50)             */
+    int32_t keep_states;       /* keep final states for cubble_server_get_state */
+    int32_t precision;         /* CUBBLE_PREC_* of every request                */
+} cubble_server_opts;
+
+typedef struct cubble_request_result {
+    int64_t id;
+    int32_t status;            /* 0 queued, 1 resident, 2 done, < 0 error code  */
+    int32_t start_count;
+    int32_t final_count;
+    int32_t pad;
+    int64_t steps;
+    double bubble_steps;
+    double arrival_s;          /* seconds from the start of cubble_server_run   */
+    double admit_s;
+    double finish_s;
+} cubble_request_result;
+
+typedef struct cubble_server_stats {
+    int64_t requests;
+    int64_t rounds;
+    int32_t max_resident;
+    int32_t pad;
+    double wall_s;
+    double bubble_steps;
+    double throughput;         /* bubble-steps/s over the run                   */
+    double p50_latency_s;      /* completion - arrival                          */
+    double p99_latency_s;
+    double peak_bytes;         /* device memory held by resident foams          */
+} cubble_server_stats;
+
+int cubble_server_create(const cubble_server_opts *opts, cubble_server **out);
+/* Queue a request arriving `arrival_s` seconds after the run starts. */
+int cubble_server_submit(cubble_server *s, const cubble_config *cfg, int64_t max_steps,
+                         double dissolve_fraction, double arrival_s, int64_t *id);
+/* Serve every queued request (arrivals in real time); returns when all are done. */
+int cubble_server_run(cubble_server *s, cubble_server_stats *stats);
+int cubble_server_result(cubble_server *s, int64_t id, cubble_request_result *out);
+int cubble_server_get_state(cubble_server *s, int64_t id, cubble_state_view *view,
+                            cubble_scalars *sc);
+const char *cubble_server_last_error(const cubble_server *s);
+void cubble_server_destroy(cubble_server *s);
+
+/* Device-memory footprint of one foam of this configuration (admission size). */
+int64_t cubble_foam_footprint(const cubble_config *cfg);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* Average device time of each kernel of one attempt, measured with HIP

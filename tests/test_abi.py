@@ -14,8 +14,8 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 def header_functions():
     text = open(ca.HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(cubble_\w+)\s*\(",
-                                 text, re.M)))
+    return sorted(set(re.findall(
+        r"^\s*(?:int|int64_t|void|const char \*)\s*\*?\s*(cubble_\w+)\s*\(", text, re.M)))
 
 
 def test_library_loads_and_exports_header_symbols():

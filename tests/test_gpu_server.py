@@ -1,0 +1,86 @@
+"""Serving loop: requests arriving while others run are admitted at round
+boundaries as memory frees up, their work overlaps on the device, and each
+request's final state equals the same foam run alone (bit for bit)."""
+import numpy as np
+import pytest
+
+import cubble_amd as ca
+from parity import config
+
+pytestmark = pytest.mark.gpu
+
+GAS = ca.IntegrationParams(True, False, True)
+
+
+def request_configs():
+    specs = [(3, 3000, 0.85), (2, 4000, 0.9), (3, 1500, 0.92), (2, 2500, 0.8), (3, 5000, 0.88),
+             (2, 1200, 0.95)]
+    return [config(d, n, phi=phi, seed=700 + k) for k, (d, n, phi) in enumerate(specs)]
+
+
+def alone(cfg, steps):
+    sim = ca.Simulator(cfg)
+    sim.run_steps(steps, GAS)
+    out = sim.get_state()
+    sim.close()
+    return out
+
+
+def test_server_results_equal_alone_and_overlap():
+    cfgs = request_configs()
+    srv = ca.Server(hbm_budget_bytes=1 << 30, round_steps=20, keep_states=True)
+    ids = [srv.submit(c, max_steps=45, arrival_s=0.03 * k) for k, c in enumerate(cfgs)]
+    stats = srv.run()
+    assert stats["requests"] == len(cfgs)
+    assert stats["max_resident"] >= 2          # requests overlapped on the device
+    assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+    for rid, cfg in zip(ids, cfgs):
+        r = srv.result(rid)
+        assert r["status"] == 2 and r["steps"] == 45
+        assert r["admit_s"] >= r["arrival_s"] and r["finish_s"] > r["admit_s"]
+        gs, gsc = srv.get_state(rid)
+        ws, wsc = alone(cfg, 45)
+        for key in gs:
+            assert np.array_equal(gs[key], ws[key]), key
+        assert gsc["dt"] == wsc["dt"] and gsc["bubble_steps"] == wsc["bubble_steps"]
+    srv.close()
+
+
+def test_server_admission_by_footprint():
+    cfgs = request_configs()[:4]
+    need = max(ca.foam_footprint(c) for c in cfgs)
+    # room for about two foams at once
+    srv = ca.Server(hbm_budget_bytes=int(2.6 * need), round_steps=15)
+    for c in cfgs:
+        srv.submit(c, max_steps=30, arrival_s=0.0)
+    stats = srv.run()
+    assert stats["requests"] == 4
+    assert stats["max_resident"] <= 2
+    assert stats["peak_bytes"] <= 2.6 * need
+    srv.close()
+
+
+def test_server_rejects_request_larger_than_budget():
+    cfg = config(3, 20000, seed=5)
+    srv = ca.Server(hbm_budget_bytes=ca.foam_footprint(cfg) // 2, round_steps=10)
+    rid = srv.submit(cfg, max_steps=10)
+    srv.run()
+    assert srv.result(rid)["status"] == ca.E_NOMEM
+    srv.close()
+
+
+def test_server_dissolution_stop():
+    # a broad size distribution with strong gas exchange dissolves bubbles quickly
+    cfg = config(2, 3000, dist=1, phi=0.9, seed=9, r_std=0.5, kappa=0.5, k_param=1.0)
+    srv = ca.Server(hbm_budget_bytes=1 << 28, round_steps=25, keep_states=True)
+    rid = srv.submit(cfg, max_steps=400, dissolve_fraction=0.01)
+    srv.run()
+    r = srv.result(rid)
+    assert r["status"] == 2
+    stop = int(np.floor(0.99 * 3000))
+    assert r["final_count"] <= stop or r["steps"] == 400
+    if r["steps"] < 400:
+        gs, _ = srv.get_state(rid)
+        ws, _ = alone(cfg, r["steps"])
+        assert np.array_equal(gs["x"], ws["x"]) and np.array_equal(gs["idx"], ws["idx"])
+    srv.close()
