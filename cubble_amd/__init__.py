@@ -108,16 +108,43 @@ class _KernelTimes(ctypes.Structure):
 
 
 class _Tuning(ctypes.Structure):
-    _fields_ = [("bubbles_per_thread", ctypes.c_int32), ("search_cells", ctypes.c_int32)]
+    _fields_ = [("bubbles_per_thread", ctypes.c_int32), ("search_cells", ctypes.c_int32),
+                ("graph_steps", ctypes.c_int32)]
+
+
+TUNING_KEYS = ("bubbles_per_thread", "search_cells", "graph_steps")
 
 
 def _tuning(t):
+    """None, a dict with TUNING_KEYS (missing = 0, the built-in default) or a
+    sequence in that order."""
     if t is None:
         return None
     if isinstance(t, dict):
-        return ctypes.byref(_Tuning(int(t.get("bubbles_per_thread", 0)),
-                                    int(t.get("search_cells", 0))))
-    return ctypes.byref(_Tuning(int(t[0]), int(t[1])))
+        return ctypes.byref(_Tuning(*[int(t.get(k, 0)) for k in TUNING_KEYS]))
+    vals = [int(v) for v in t] + [0] * (3 - len(t))
+    return ctypes.byref(_Tuning(*vals[:3]))
+
+
+class _AutotuneOpts(ctypes.Structure):
+    _fields_ = [("trial_steps", ctypes.c_int32), ("repeats", ctypes.c_int32),
+                ("ignore_cache", ctypes.c_int32), ("cache_path", ctypes.c_char_p)]
+
+
+class _AutotuneReport(ctypes.Structure):
+    _fields_ = [("chosen", _Tuning), ("from_cache", ctypes.c_int32),
+                ("candidates_timed", ctypes.c_int32), ("start_rate", ctypes.c_double),
+                ("best_rate", ctypes.c_double)]
+
+
+def _autotune(fn, h, ip, trial_steps, repeats, cache_path, ignore_cache, check):
+    o = _AutotuneOpts(int(trial_steps), int(repeats), int(bool(ignore_cache)),
+                      cache_path.encode() if cache_path else None)
+    r = _AutotuneReport()
+    check(fn(h, ctypes.byref((ip or IntegrationParams())._c()), ctypes.byref(o), ctypes.byref(r)))
+    return {"chosen": {k: getattr(r.chosen, k) for k in TUNING_KEYS},
+            "from_cache": bool(r.from_cache), "candidates_timed": r.candidates_timed,
+            "start_rate": r.start_rate, "best_rate": r.best_rate}
 
 
 # exported symbols of include/cubble_amd.h (checked by the ABI test)
@@ -132,7 +159,8 @@ EXPORTS = [
     "cubble_batch_last_error", "cubble_batch_destroy", "cubble_batch_time_kernels",
     "cubble_server_create", "cubble_server_submit", "cubble_server_run", "cubble_server_result",
     "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
-    "cubble_foam_footprint",
+    "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
+    "cubble_batch_retune",
 ]
 
 
@@ -215,6 +243,11 @@ def lib():
         L.cubble_server_destroy.argtypes = [vp]
         L.cubble_foam_footprint.argtypes = [P(_Config)]
         L.cubble_foam_footprint.restype = ctypes.c_int64
+        for name in ("cubble_autotune", "cubble_batch_autotune"):
+            getattr(L, name).argtypes = [vp, P(_IntegrateParams), P(_AutotuneOpts),
+                                         P(_AutotuneReport)]
+        for name in ("cubble_retune", "cubble_batch_retune"):
+            getattr(L, name).argtypes = [vp, P(_Tuning)]
         _lib = L
     return _lib
 
@@ -381,6 +414,15 @@ class Simulator:
                                           int(n), ctypes.byref(r)))
         return _res(r)
 
+    def autotune(self, ip=None, trial_steps=32, repeats=2, cache_path=None, ignore_cache=False):
+        """Pick launch shapes by timing them on this foam; the state is
+        restored afterwards.  Returns the chosen shapes and the rates."""
+        return _autotune(lib().cubble_autotune, self._h, ip, trial_steps, repeats, cache_path,
+                         ignore_cache, self._call)
+
+    def retune(self, tuning):
+        self._call(lib().cubble_retune(self._h, _tuning(tuning)))
+
     def time_kernels(self, reps=20, ip=None):
         """Average device ms of predictor / pair / corrector, from HIP events."""
         t = _KernelTimes()
@@ -533,6 +575,15 @@ class Batch:
     def get_state(self, k):
         sc = self.scalars(k)
         return _read_state(sc, lambda v: self._call(lib().cubble_batch_get_state(self._h, int(k), v))), sc
+
+    def autotune(self, ip=None, trial_steps=32, repeats=2, cache_path=None, ignore_cache=False):
+        """Pick launch shapes by timing them on these foams; the state is
+        restored afterwards.  Returns the chosen shapes and the rates."""
+        return _autotune(lib().cubble_batch_autotune, self._h, ip, trial_steps, repeats,
+                         cache_path, ignore_cache, self._call)
+
+    def retune(self, tuning):
+        self._call(lib().cubble_batch_retune(self._h, _tuning(tuning)))
 
     def time_kernels(self, reps=20, ip=None):
         t = _KernelTimes()

@@ -77,23 +77,33 @@ size_t carve_foam(Foam &F, char *base) {
     S.cell_count = L.take<int32_t>((size_t)F.cell_cap);
     S.cell_cursor = L.take<int32_t>((size_t)F.cell_cap);
     S.cell_start = L.take<int32_t>((size_t)F.cell_cap + 1);
-    S.part = L.take<double>((size_t)F.nblk * kPartD);
-    S.part_i = L.take<int32_t>((size_t)F.nblk * kPartI);
+    // block partials sized for the smallest launch shapes (one bubble per
+    // thread, kMinSearchCells per search block), so retuning never reallocates
+    const size_t nblk_max = ((size_t)F.cap + kBlock - 1) / kBlock;
+    const size_t sblk_max = ((size_t)F.cell_cap + kMinSearchCells - 1) / kMinSearchCells;
+    S.part = L.take<double>(nblk_max * kPartD);
+    S.part_i = L.take<int32_t>(nblk_max * kPartI);
     S.cpart = L.take<int32_t>((size_t)F.cblk);
-    S.cpart_i = L.take<int32_t>((size_t)F.sblk * kPartI);
+    S.cpart_i = L.take<int32_t>(sblk_max * kPartI);
     return L.off;
+}
+
+// Launch shapes: bubble-space blocks and neighbour-search tiles.
+void shape_foam(Foam &F, const Tuning &t) {
+    const int cap = F.cap;
+    F.ipt = t.ipt > 0 ? t.ipt : std::max(1, (cap + kBlock * kMaxBlocks - 1) / (kBlock * kMaxBlocks));
+    F.nblk = std::max(1, (cap + kBlock * F.ipt - 1) / (kBlock * F.ipt));
+    F.scells = std::max(kMinSearchCells, t.scells > 0 ? t.scells : 32);
+    F.sblk = std::max(1, (F.cell_cap + F.scells - 1) / F.scells);
 }
 
 void size_foam(Foam &F, int cap, int nbr_cap, int cell_cap, const Tuning &t) {
     F.cap = cap;
     F.nbr_cap = nbr_cap;
     F.cell_cap = cell_cap;
-    F.ipt = t.ipt > 0 ? t.ipt : std::max(1, (cap + kBlock * kMaxBlocks - 1) / (kBlock * kMaxBlocks));
-    F.nblk = std::max(1, (cap + kBlock * F.ipt - 1) / (kBlock * F.ipt));
     F.cipt = std::max(1, (cell_cap + kBlock * 1024 - 1) / (kBlock * 1024));
     F.cblk = std::max(1, (cell_cap + kBlock * F.cipt - 1) / (kBlock * F.cipt));
-    F.scells = t.scells > 0 ? t.scells : 32;
-    F.sblk = std::max(1, (cell_cap + F.scells - 1) / F.scells);
+    shape_foam(F, t);
 }
 
 int default_nbr_cap(int dim) { return 3 == dim ? 48 : 24; }
@@ -261,6 +271,7 @@ int Engine::add_generated(const cubble_config &cfg, const Tuning &t, const std::
     d.need_search = 1;
     r.num_end = cfg.num_end;
     r.max_steps = cfg.max_steps;
+    tuning_ = t;
     foams.push_back(r);
     dev_index_.push_back(-1);
     if (index) *index = (int)foams.size() - 1;
@@ -270,14 +281,9 @@ int Engine::add_generated(const cubble_config &cfg, const Tuning &t, const std::
 int Engine::drop_foam(int k) {
     if (k < 0 || k >= (int)foams.size()) return fail(CUBBLE_E_INVALID, "no such foam");
     // keep the device state of the remaining foams before the array changes
-    if (dfoams_) {
-        E_TRY(hipStreamSynchronize(stream));
-        for (size_t i = 0; i < foams.size(); ++i)
-            if (dev_index_[i] >= 0)
-                E_TRY(hipMemcpy(&foams[i].h, dfoams_ + dev_index_[i], sizeof(Foam),
-                                hipMemcpyDeviceToHost));
-        for (auto &di : dev_index_) di = -2;  // host mirrors are current
-    }
+    const int rc = pull_headers();
+    if (rc) return rc;
+    for (auto &di : dev_index_) di = -2;  // host mirrors are current
     arena.release(foams[k].block);
     arena.release(foams[k].nbr_block);
     foams.erase(foams.begin() + k);
@@ -285,14 +291,26 @@ int Engine::drop_foam(int k) {
     return CUBBLE_OK;
 }
 
+int Engine::pull_headers() {
+    if (!dfoams_) return CUBBLE_OK;
+    E_TRY(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < foams.size(); ++i)
+        if (dev_index_[i] >= 0)
+            E_TRY(hipMemcpy(&foams[i].h, dfoams_ + dev_index_[i], sizeof(Foam),
+                            hipMemcpyDeviceToHost));
+    return CUBBLE_OK;
+}
+
 int Engine::commit_layout() {
     E_TRY(hipStreamSynchronize(stream));
     const int nf = (int)foams.size();
     // device state of foams already on the device is the truth
-    for (int i = 0; i < nf; ++i)
-        if (dev_index_[i] >= 0 && dfoams_)
-            E_TRY(hipMemcpy(&foams[i].h, dfoams_ + dev_index_[i], sizeof(Foam),
-                            hipMemcpyDeviceToHost));
+    int rc = pull_headers();
+    if (rc) return rc;
+    const Foam *old_foams = dfoams_;
+    const int2 *old_bt[3] = {bt_bub_, bt_cell_, bt_scell_};
+    const int old_nb[3] = {nb_bub_, nb_cell_, nb_scell_};
+    const int old_nf = committed_nf_;
     if (nf > dfoams_cap_) {
         if (dfoams_) (void)hipFree(dfoams_);
         dfoams_ = nullptr;
@@ -324,18 +342,93 @@ int Engine::commit_layout() {
         if (!v.empty()) E_TRY(hipMemcpy(buf, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice));
         return CUBBLE_OK;
     };
-    int rc;
     if ((rc = upload(bt_bub_, bt_bub_cap_, bb)) || (rc = upload(bt_cell_, bt_cell_cap_, bc)) ||
         (rc = upload(bt_scell_, bt_scell_cap_, bs)))
         return rc;
     nb_bub_ = (int)bb.size();
     nb_cell_ = (int)bc.size();
     nb_scell_ = (int)bs.size();
-    if (graph_) {
+    // the captured graph holds the header array, the block tables and the
+    // grid sizes; it stays valid while none of them moved or changed size
+    const bool same = old_foams == dfoams_ && old_bt[0] == bt_bub_ && old_bt[1] == bt_cell_ &&
+                      old_bt[2] == bt_scell_ && old_nb[0] == nb_bub_ && old_nb[1] == nb_cell_ &&
+                      old_nb[2] == nb_scell_ && old_nf == nf;
+    committed_nf_ = nf;
+    if (graph_ && !same) {
         (void)hipGraphExecDestroy(graph_);
         graph_ = nullptr;
     }
     return CUBBLE_OK;
+}
+
+int Engine::retune(const Tuning &t) {
+    int rc = pull_headers();
+    if (rc) return rc;
+    for (auto &di : dev_index_) di = -2;  // host mirrors are current
+    tuning_ = t;
+    for (auto &r : foams) shape_foam(r.h, t);
+    return commit_layout();
+}
+
+void Engine::free_snapshot(Snapshot &s) {
+    for (void *p : s.blk)
+        if (p) (void)hipFree(p);
+    for (void *p : s.nbr)
+        if (p) (void)hipFree(p);
+    s = Snapshot();
+}
+
+int Engine::snapshot(Snapshot &s) {
+    free_snapshot(s);
+    int rc = pull_headers();
+    if (rc) return rc;
+    for (const FoamRec &r : foams) {
+        Foam tmp = r.h;
+        const size_t bb = carve_foam(tmp, nullptr);
+        const size_t nb = (size_t)r.h.nbr_cap * r.h.cap * sizeof(int32_t);
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, bb) != hipSuccess || hipMalloc(&b, nb) != hipSuccess) {
+            if (a) (void)hipFree(a);
+            free_snapshot(s);
+            return fail(CUBBLE_E_NOMEM, "no device memory for a state snapshot");
+        }
+        s.h.push_back(r.h);
+        s.blk.push_back(a);
+        s.nbr.push_back(b);
+        s.blk_bytes.push_back(bb);
+        s.nbr_bytes.push_back(nb);
+        E_TRY(hipMemcpyAsync(a, r.block, bb, hipMemcpyDeviceToDevice, stream));
+        E_TRY(hipMemcpyAsync(b, r.nbr_block, nb, hipMemcpyDeviceToDevice, stream));
+    }
+    E_TRY(hipStreamSynchronize(stream));
+    return CUBBLE_OK;
+}
+
+int Engine::restore(const Snapshot &s) {
+    if (s.h.size() != foams.size()) return fail(CUBBLE_E_STATE, "snapshot of other foams");
+    int rc = pull_headers();
+    if (rc) return rc;
+    for (size_t k = 0; k < foams.size(); ++k) {
+        FoamRec &r = foams[k];
+        // lists only grow, and the first columns of the grown block hold the
+        // saved ones (column q of bubble i is nbr[q * cap + i])
+        if (s.h[k].nbr_cap > r.h.nbr_cap || s.h[k].cap != r.h.cap)
+            return fail(CUBBLE_E_STATE, "snapshot does not fit the foam");
+        E_TRY(hipMemcpyAsync(r.block, s.blk[k], s.blk_bytes[k], hipMemcpyDeviceToDevice, stream));
+        E_TRY(hipMemcpyAsync(r.nbr_block, s.nbr[k], s.nbr_bytes[k], hipMemcpyDeviceToDevice,
+                             stream));
+        Foam f = s.h[k];
+        f.s.nbr = r.h.s.nbr;
+        f.nbr_cap = r.h.nbr_cap;
+        f.ipt = r.h.ipt;
+        f.nblk = r.h.nblk;
+        f.scells = r.h.scells;
+        f.sblk = r.h.sblk;
+        r.h = f;
+    }
+    E_TRY(hipStreamSynchronize(stream));
+    for (auto &di : dev_index_) di = -2;
+    return commit_layout();
 }
 
 Grid Engine::grid() const {
@@ -512,7 +605,7 @@ int Engine::integrate(uint32_t flags, std::vector<Dyn> &d) {
 int Engine::run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d,
                       const std::vector<int64_t> *per_foam) {
     if (steps <= 0) return read_dyn_all(d);
-    const int groups = 16;
+    const int groups = std::max(1, tuning_.groups);
     int rc = capture(flags, groups);
     if (rc) return rc;
     if ((rc = set_limits(steps, per_foam))) return rc;

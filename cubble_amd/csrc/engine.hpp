@@ -47,6 +47,18 @@ class DeviceArena {
 struct Tuning {
     int ipt = 0;       // bubbles per thread (0 = from the size)
     int scells = 32;   // cells per neighbour-search block
+    int groups = 16;   // attempt groups per captured graph replay
+};
+
+// Smallest search tile: the search block has four waves, one cell each.
+constexpr int kMinSearchCells = 4;
+
+// Device copies of every foam's buffers and host copies of their headers,
+// taken by Engine::snapshot and put back by Engine::restore.
+struct Snapshot {
+    std::vector<Foam> h;
+    std::vector<void *> blk, nbr;
+    std::vector<size_t> blk_bytes, nbr_bytes;
 };
 
 // Host-side record of one foam.
@@ -109,6 +121,15 @@ class Engine {
     // Step limits for the next run (0 = none); clears halt and overflow.
     int set_limits(int64_t steps, const std::vector<int64_t> *per_foam);
 
+    // New launch shapes for every foam (within the block partials carved for
+    // the smallest shapes); the state is untouched.
+    int retune(const Tuning &t);
+    const Tuning &tuning() const { return tuning_; }
+    // Copy the whole device state aside / put it back (launch shapes stay).
+    int snapshot(Snapshot &s);
+    int restore(const Snapshot &s);
+    void free_snapshot(Snapshot &s);
+
     int fail(int code, const std::string &m) {
         err = m;
         return code;
@@ -123,6 +144,8 @@ class Engine {
 
   private:
     int grow_lists(int k, int need, bool rebuild);
+    int pull_headers();
+    Tuning tuning_;
     Foam *dfoams_ = nullptr;
     int dfoams_cap_ = 0;
     int2 *bt_bub_ = nullptr, *bt_cell_ = nullptr, *bt_scell_ = nullptr;
@@ -134,6 +157,7 @@ class Engine {
     hipGraphExec_t graph_ = nullptr;
     uint32_t graph_flags_ = 0;
     int graph_groups_ = 0;
+    int committed_nf_ = 0;  // foams in the device array at the last commit
 
   public:
     Foam *dfoams() const { return dfoams_; }

@@ -437,4 +437,76 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
     return CUBBLE_OK;
 }
 
+namespace {
+
+bool read_cache(const char *path, JVal &root) {
+    std::ifstream f(path);
+    if (!f.is_open()) return false;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    JParser jp(text);
+    return jp.val(root) && root.kind == JVal::Obj;
+}
+
+}  // namespace
+
+bool tuning_cache_lookup(const char *path, const std::string &key, int shape[3]) {
+    JVal root;
+    if (!path || !read_cache(path, root)) return false;
+    const JVal *e = root.get(key.c_str());
+    if (!e) return false;
+    double v[3];
+    if (!jnum(*e, {"bubbles_per_thread"}, v[0]) || !jnum(*e, {"search_cells"}, v[1]) ||
+        !jnum(*e, {"graph_steps"}, v[2]))
+        return false;
+    for (int k = 0; k < 3; ++k) shape[k] = (int)v[k];
+    return true;
+}
+
+int tuning_cache_store(const char *path, const std::string &key, const int shape[3], double rate,
+                       std::string &err) {
+    // keep the other entries; an unreadable file is replaced
+    JVal root;
+    std::map<std::string, std::string> entries;
+    if (read_cache(path, root)) {
+        for (const auto &kv : root.obj) {
+            double v[4] = {0, 0, 0, 0};
+            if (!jnum(kv.second, {"bubbles_per_thread"}, v[0]) ||
+                !jnum(kv.second, {"search_cells"}, v[1]) || !jnum(kv.second, {"graph_steps"}, v[2]))
+                continue;
+            jnum(kv.second, {"rate"}, v[3]);
+            char buf[256];
+            std::snprintf(buf, sizeof(buf),
+                          "{\"bubbles_per_thread\": %d, \"search_cells\": %d, \"graph_steps\": %d, "
+                          "\"rate\": %.6g}",
+                          (int)v[0], (int)v[1], (int)v[2], v[3]);
+            entries[kv.first] = buf;
+        }
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof(buf),
+                  "{\"bubbles_per_thread\": %d, \"search_cells\": %d, \"graph_steps\": %d, "
+                  "\"rate\": %.6g}",
+                  shape[0], shape[1], shape[2], rate);
+    entries[key] = buf;
+    const std::string tmp = std::string(path) + ".tmp";
+    std::ofstream f(tmp);
+    if (!f.is_open()) {
+        err = std::string("couldn't write tuning cache ") + path;
+        return CUBBLE_E_IO;
+    }
+    f << "{\n";
+    size_t k = 0;
+    for (const auto &kv : entries)
+        f << "  \"" << kv.first << "\": " << kv.second << (++k < entries.size() ? ",\n" : "\n");
+    f << "}\n";
+    f.close();
+    if (!f || std::rename(tmp.c_str(), path) != 0) {
+        err = std::string("couldn't write tuning cache ") + path;
+        return CUBBLE_E_IO;
+    }
+    return CUBBLE_OK;
+}
+
 }  // namespace cubble_amd

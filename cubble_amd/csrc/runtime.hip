@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "autotune.hpp"
 #include "cubble_amd.h"
 #include "engine.hpp"
 #include "foam.hpp"
@@ -59,8 +60,16 @@ Tuning tuning_of(const cubble_tuning *t) {
     if (t) {
         u.ipt = t->bubbles_per_thread;
         if (t->search_cells > 0) u.scells = t->search_cells;
+        if (t->graph_steps > 0) u.groups = t->graph_steps;
     }
     return u;
+}
+
+int retune(Engine &e, const cubble_tuning *t) {
+    if (!t || t->bubbles_per_thread < 0 || t->search_cells < 0 || t->graph_steps < 0 ||
+        (t->search_cells > 0 && t->search_cells < kMinSearchCells))
+        return e.fail(CUBBLE_E_INVALID, "bad launch shapes");
+    return e.retune(tuning_of(t));
 }
 
 void fill_result(const Dyn &d, cubble_step_result *r) {
@@ -184,6 +193,17 @@ int cubble_create_tuned(const cubble_config *cfg, const cubble_tuning *tuning, c
     }
     *out = s;
     return CUBBLE_OK;
+}
+
+int cubble_autotune(cubble_sim *sim, const cubble_integrate_params *ip,
+                    const cubble_autotune_opts *opts, cubble_autotune_report *report) {
+    if (!sim) return CUBBLE_E_INVALID;
+    const int rc = check_ip(sim->e, ip);
+    return rc ? rc : autotune(sim->e, flags_of(ip) | kCommit, opts, report);
+}
+
+int cubble_retune(cubble_sim *sim, const cubble_tuning *tuning) {
+    return sim ? retune(sim->e, tuning) : CUBBLE_E_INVALID;
 }
 
 int cubble_create(const cubble_config *cfg, cubble_sim **out) {
@@ -377,6 +397,17 @@ int cubble_batch_get_state(cubble_batch *b, int32_t k, cubble_state_view *view) 
 int cubble_batch_time_kernels(cubble_batch *b, const cubble_integrate_params *ip, int32_t reps,
                               cubble_kernel_times *out) {
     return b ? time_kernels(b->e, ip, reps, out) : CUBBLE_E_INVALID;
+}
+
+int cubble_batch_autotune(cubble_batch *b, const cubble_integrate_params *ip,
+                          const cubble_autotune_opts *opts, cubble_autotune_report *report) {
+    if (!b) return CUBBLE_E_INVALID;
+    const int rc = check_ip(b->e, ip);
+    return rc ? rc : autotune(b->e, flags_of(ip) | kCommit, opts, report);
+}
+
+int cubble_batch_retune(cubble_batch *b, const cubble_tuning *tuning) {
+    return b ? retune(b->e, tuning) : CUBBLE_E_INVALID;
 }
 
 const char *cubble_batch_last_error(const cubble_batch *b) {

@@ -207,12 +207,38 @@ int cubble_set_state(cubble_sim *sim, const cubble_state_view *view, const cubbl
  * No counterpart in the reference (its launches are fixed at 5120 x 384,
  * src/data_definitions.h:28-29). */
 typedef struct cubble_tuning {
-    int32_t bubbles_per_thread;  /* bubble-space kernels                     */
-    int32_t search_cells;        /* cells per neighbour-search block          */
+    int32_t bubbles_per_thread;  /* bubble-space kernels (0 = from the foam size)   */
+    int32_t search_cells;        /* cells per neighbour-search block (0 = 32, >= 4) */
+    int32_t graph_steps;         /* attempt groups per graph replay (0 = 16)        */
 } cubble_tuning;
 
 /* cubble_create with explicit launch shapes. */
 int cubble_create_tuned(const cubble_config *cfg, const cubble_tuning *tuning, cubble_sim **out);
+
+/* Autotuner options and report.  The tuner times candidate launch shapes on
+ * the handle's own foams, each from the same copy of the device state, keeps
+ * the fastest, and restores the state (tuning does not advance the
+ * simulation).  With cache_path, shapes are looked up in / stored to a JSON
+ * file keyed by device and batch make-up.  No counterpart in the reference. */
+typedef struct cubble_autotune_opts {
+    int32_t trial_steps;    /* accepted steps timed per candidate (0 = 32)   */
+    int32_t repeats;        /* timings per candidate, fastest kept (0 = 2)   */
+    int32_t ignore_cache;   /* time the candidates even on a cache hit       */
+    const char *cache_path; /* JSON cache file or NULL                       */
+} cubble_autotune_opts;
+
+typedef struct cubble_autotune_report {
+    cubble_tuning chosen;
+    int32_t from_cache;
+    int32_t candidates_timed;
+    double start_rate;      /* bubble-steps/s with the shapes in use before */
+    double best_rate;       /* with the chosen shapes (0 on a cache hit)    */
+} cubble_autotune_report;
+
+int cubble_autotune(cubble_sim *sim, const cubble_integrate_params *ip,
+                    const cubble_autotune_opts *opts, cubble_autotune_report *report);
+/* Apply launch shapes to a live simulation (state untouched). */
+int cubble_retune(cubble_sim *sim, const cubble_tuning *tuning);
 
 /* ---- batched foams --------------------------------------------------------- */
 
@@ -238,6 +264,9 @@ int cubble_batch_run_steps(cubble_batch *b, const cubble_integrate_params *ip, i
 int cubble_batch_get_scalars(cubble_batch *b, int32_t k, cubble_scalars *out);
 int cubble_batch_get_state(cubble_batch *b, int32_t k, cubble_state_view *view);
 const char *cubble_batch_last_error(const cubble_batch *b);
+int cubble_batch_autotune(cubble_batch *b, const cubble_integrate_params *ip,
+                          const cubble_autotune_opts *opts, cubble_autotune_report *report);
+int cubble_batch_retune(cubble_batch *b, const cubble_tuning *tuning);
 void cubble_batch_destroy(cubble_batch *b);
 
 /* ---- serving ----------------------------------------------------------------- */

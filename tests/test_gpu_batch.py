@@ -92,3 +92,52 @@ def test_batch_rejects_mixed_precision_modes():
     cfgs = [config(2, 500, seed=1), config(2, 500, seed=2, precision=ca.PREC_MIXED)]
     with pytest.raises(ca.CubbleError):
         ca.Batch(cfgs)
+
+
+def test_autotune_restores_state_and_chosen_shapes_match_alone(tmp_path):
+    cfgs = mixed_configs()[:4]
+    batch = ca.Batch(cfgs)
+    batch.run_steps(5, GAS)
+    before = [batch.get_state(k) for k in range(len(cfgs))]
+    cache = str(tmp_path / "tuning.json")
+    rep = batch.autotune(GAS, trial_steps=16, repeats=1, cache_path=cache)
+    assert rep["candidates_timed"] >= 10 and not rep["from_cache"]
+    assert rep["best_rate"] >= rep["start_rate"] > 0
+    # tuning times real steps but hands the foams back unchanged
+    for k in range(len(cfgs)):
+        assert_same(batch.get_state(k), before[k])
+    chosen = rep["chosen"]
+    batch.run_steps(30, GAS)
+    for k in (0, 3):
+        # same shapes step for step: 5 steps with the defaults, then the chosen
+        sim = ca.Simulator(cfgs[k])
+        sim.run_steps(5, GAS)
+        sim.retune(chosen)
+        sim.run_steps(30, GAS)
+        assert_same(batch.get_state(k), sim.get_state())
+        sim.close()
+    # same batch make-up: the cache answers without timing
+    other = ca.Batch(cfgs)
+    rep2 = other.autotune(GAS, cache_path=cache)
+    assert rep2["from_cache"] and rep2["chosen"] == chosen and rep2["candidates_timed"] == 0
+    other.close()
+    batch.close()
+
+
+def test_retune_live_simulation_equals_created_with_shapes():
+    cfg = config(3, 30000, seed=21)
+    shapes = {"bubbles_per_thread": 4, "search_cells": 8, "graph_steps": 4}
+    a = ca.Simulator(cfg)
+    a.run_steps(10, GAS)
+    a.retune(shapes)
+    b = ca.Simulator(cfg, tuning=shapes)
+    b.retune({"bubbles_per_thread": 0, "search_cells": 32, "graph_steps": 16})
+    b.run_steps(10, GAS)
+    b.retune(shapes)
+    a.run_steps(20, GAS)
+    b.run_steps(20, GAS)
+    assert_same(a.get_state(), b.get_state())
+    with pytest.raises(ca.CubbleError):
+        a.retune({"search_cells": 2})
+    a.close()
+    b.close()
