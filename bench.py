@@ -1,14 +1,29 @@
-"""Benchmark: bubble-steps/s of the per-timestep foam path on one MI355X.
+"""Benchmark of the per-timestep foam path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload foam3d_1e6] [--precision fp64]
+    python bench.py [--steps K] [--warmup W] [--legs batch,plain,foam,serve]
 
 A "step" is one accepted integrate() call -- predictor, pairwise forces and
 gas exchange, corrector, adaptive dt, removal of dissolved bubbles and the
-neighbour search when triggered -- over one synthetic foam resident in HBM.
-value = live bubbles summed over the K timed accepted steps / wall time.
-Prints one JSON line (rank 0, N = 1).
+neighbour search when triggered -- of every foam of the workload, with the
+foams resident in HBM.  Throughput is live bubbles summed over accepted steps
+per second of wall time.
+
+The headline (`value`) is the batched mixed-precision form: 256 independent
+foams (half 2D, half 3D, 1e3..1e5 bubbles each) advanced together by replayed
+HIP graphs with autotuned launch shapes, K steps each.  The same line carries:
+  plain_fp64_same_foams  the same 256 foams, FP64, each run alone, one after
+                         another (the plain path), and the speedup over it;
+  foam3d_1e6_fp64        one 3D foam of 1e6 bubbles, FP64, 2000 steps, with
+                         the roofline of its dominant kernel;
+  serving                Poisson requests (50/s, 3D, 1e3..1e6 bubbles, 500
+                         steps or half dissolved) through the continuous-
+                         batching request loop: p50 / p99 latency, throughput;
+  cpu_baseline           the OpenMP FP64 restatement on the host cores, on a
+                         bounded sample of the headline workload.
+Prints one JSON line.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -20,68 +35,310 @@ sys.path.insert(0, ROOT)
 METRIC = "bubble-steps/s mixed+batched vs plain FP64 on GPU; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
 
-WORKLOADS = {
-    # name: (description, default steps)
-    "foam2d_1e4": ("2D periodic foam, 10,000 bubbles, clipped-normal radii, phi 0.9", 1000),
-    "foam3d_1e6": ("3D periodic foam, 1,000,000 bubbles, log-normal radii sigma 0.15, phi 0.84",
-                   2000),
-    "foam3d_1e7": ("3D periodic foam, 10,000,000 bubbles, log-normal radii sigma 0.15, phi 0.9",
-                   500),
-}
+
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def kernel_bytes(dim, p):
     """Algorithmic (compulsory) HBM bytes per bubble of each kernel of an
-    attempt, FP64 storage; p = half-list pairs per bubble."""
+    attempt; p = half-list pairs per bubble (each bubble reads 2p list
+    entries).  State is FP64 in both precision modes."""
     c = dim + 1
     return {
-        # read x, r, v, v_old; write x_p, r_p
+        # read x, r, v, v_old; write the predicted {x, y, z, r} record
         "predict": 8 * 4 * c,
-        # read x_p, r_p, neighbour count, 2p neighbour indices; write v_p (incl. dR/dt part), overlap
+        # read own record, neighbour count, 2p neighbour indices (the gathered
+        # neighbour records are other bubbles' own records: L2 hits when the
+        # lists are cell ordered); write v_p incl. the pair dR/dt, overlap
         "pair": 8 * c + 4 + 8 * p + 8 * c + 8,
-        # read x_p, x, v, v_p, saved (c each), overlap, path, error; write x_c, dR/dt, path_new,
-        # error (+1 byte deletion mark)
+        # read predicted record, x, v, v_p, saved (c each), overlap, path, error;
+        # write corrected x, dR/dt, path, error (+1 byte deletion mark)
         "correct": 8 * (5 * c + 3) + 8 * (c + 3) + 1,
     }
 
 
 def step_bytes(dim, p):
-    """Compulsory bytes per bubble-step of the whole step (read once, write once):
-    read x,r, v, v_old, saved, path, error; write x,r, v_new, path, error, plus
-    the 2p neighbour indices."""
+    """Compulsory bytes per bubble-step of a whole accepted step: read x,r, v,
+    v_old, saved, path, error; write x,r, v_new, path, error; the 2p list
+    entries."""
     c = dim + 1
     return 8 * (6 * c + 4) + 8 * p
 
 
-def cpu_baseline(cfg, budget_s):
+KERNEL_NAMES = {"predict": "k_predict", "pair": "k_pair", "correct": "k_correct"}
+
+
+def kernel_name(part, mixed):
+    return "k_pair_mixed" if part == "pair" and mixed else KERNEL_NAMES[part]
+
+
+def roofline(kt, per_foam, mixed=False):
+    """Roofline of the dominant kernel of an attempt.  kt: kernel times (ms) of
+    one launch over all foams; per_foam: [(dim, count, pairs)]."""
+    times = {"predict": kt["predict_ms"], "pair": kt["pair_ms"], "correct": kt["correct_ms"]}
+    dom = max(times, key=times.get)
+    nbytes = 0.0
+    bubbles = 0
+    for dim, n, pairs in per_foam:
+        nbytes += kernel_bytes(dim, pairs / max(n, 1))[dom] * n
+        bubbles += n
+    achieved = nbytes / (times[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": kernel_name(dom, mixed), "bytes_per_launch": round(nbytes),
+            "bubbles_per_launch": bubbles,
+            "launch_ms": round(times[dom], 4),
+            "kernels_ms": {k: round(v, 4) for k, v in times.items()}}
+
+
+def counters(before, after):
+    keys = ("attempts", "searches", "removals")
+    return {k: int(sum(a[k] for a in after) - sum(b[k] for b in before)) for k in keys}
+
+
+def leg_batch(ca, args, ip):
+    cfgs = ca.batch_workload(256, precision=ca.PREC_MIXED)
+    t = time.perf_counter()
+    b = ca.Batch(cfgs)
+    log("batch: created 256 foams (%d bubbles) in %.1f s" %
+        (sum(c.num_start for c in cfgs), time.perf_counter() - t))
+    t = time.perf_counter()
+    if args.tuning:
+        # fixed shapes (no timing trials): bubbles_per_thread,search_cells,graph_steps
+        shapes = dict(zip(ca.TUNING_KEYS, [int(v) for v in args.tuning.split(",")]))
+        b.retune(shapes)
+        tune = {"chosen": shapes, "from_cache": False, "candidates_timed": 0, "fixed": True}
+    else:
+        tune = b.autotune(ip, trial_steps=args.tune_steps, repeats=2,
+                          cache_path=args.tuning_cache)
+    log("batch: launch shapes in %.1f s: %s" % (time.perf_counter() - t, json.dumps(tune)))
+    if args.warmup > 0:
+        b.run_steps(args.warmup, ip)
+    s0 = [b.scalars(k) for k in range(len(cfgs))]
+    t0 = time.perf_counter()
+    res = b.run_steps(args.steps, ip)
+    elapsed = time.perf_counter() - t0
+    s1 = [b.scalars(k) for k in range(len(cfgs))]
+    work = sum(x["bubble_steps"] for x in s1) - sum(x["bubble_steps"] for x in s0)
+    assert all(r["steps"] >= args.steps for r in res)
+    kt = b.time_kernels(args.kernel_reps, ip)
+    rf = roofline(kt, [(x["dim"], x["count"], x["num_pairs"]) for x in s1], mixed=True)
+    live = sum(x["count"] for x in s1)
+    pairs = sum(x["num_pairs"] for x in s1)
+    rf["step_bytes_per_bubble"] = round(
+        sum(step_bytes(x["dim"], x["num_pairs"] / max(x["count"], 1)) * x["count"] for x in s1) /
+        max(live, 1), 2)
+    b.close()
+    log("batch: %d steps in %.3f s -> %.4g bubble-steps/s" % (args.steps, elapsed, work / elapsed))
+    return {"value": work / elapsed, "elapsed_s": elapsed, "bubbles_start": sum(c.num_start for c in cfgs),
+            "bubbles_end": live, "pairs_per_bubble": round(pairs / max(live, 1), 3),
+            "tuning": tune, "counters": counters(s0, s1), "roofline": rf}, cfgs
+
+
+def leg_plain(ca, args, ip, cfgs):
+    """The same foams in FP64, each alone, one after another (creation and
+    the first search untimed)."""
+    work = 0.0
+    elapsed = 0.0
+    for k, c in enumerate(cfgs):
+        c = dataclasses.replace(c, precision=ca.PREC_FP64)
+        sim = ca.Simulator(c)
+        if args.warmup > 0:
+            sim.run_steps(args.warmup, ip)
+        b0 = sim.scalars()["bubble_steps"]
+        t0 = time.perf_counter()
+        sim.run_steps(args.steps, ip)
+        elapsed += time.perf_counter() - t0
+        work += sim.scalars()["bubble_steps"] - b0
+        sim.close()
+    log("plain: 256 foams alone in %.2f s -> %.4g bubble-steps/s" % (elapsed, work / elapsed))
+    return {"value": work / elapsed, "elapsed_s": elapsed, "precision": "fp64",
+            "how": "each foam alone (cubble_create + cubble_run_steps), one after another"}
+
+
+def leg_foam(ca, args, ip):
+    cfg = ca.Config.workload("foam3d_1e6")
+    t = time.perf_counter()
+    sim = ca.Simulator(cfg)
+    log("foam3d_1e6: created in %.1f s" % (time.perf_counter() - t))
+    if args.warmup > 0:
+        sim.run_steps(args.warmup, ip)
+    r0 = sim.scalars()
+    t0 = time.perf_counter()
+    sim.run_steps(args.foam_steps, ip)
+    elapsed = time.perf_counter() - t0
+    r1 = sim.scalars()
+    work = r1["bubble_steps"] - r0["bubble_steps"]
+    kt = sim.time_kernels(args.kernel_reps, ip)
+    rf = roofline(kt, [(3, r1["count"], r1["num_pairs"])])
+    p = r1["num_pairs"] / max(r1["count"], 1)
+    rf["step_bytes_per_bubble"] = round(step_bytes(3, p), 2)
+    rf["step_gbs"] = round(step_bytes(3, p) * work / elapsed / 1e9, 1)
+    sim.close()
+    log("foam3d_1e6: %d steps in %.3f s -> %.4g bubble-steps/s" %
+        (args.foam_steps, elapsed, work / elapsed))
+    return {"value": work / elapsed, "steps": args.foam_steps,
+            "ms_per_step": elapsed / args.foam_steps * 1e3, "precision": "fp64",
+            "pairs_per_bubble": round(p, 3), "final_count": r1["count"],
+            "counters": counters([r0], [r1]), "roofline": rf}
+
+
+def leg_serve(ca, args, ip):
+    reqs = ca.serving_requests(args.serve_requests, rate_per_s=50.0, precision=ca.PREC_MIXED)
+    srv = ca.Server(hbm_budget_bytes=args.serve_budget_gib << 30, round_steps=args.serve_round,
+                    precision=ca.PREC_MIXED)
+    for cfg, at in reqs:
+        srv.submit(cfg, max_steps=500, dissolve_fraction=0.5, arrival_s=at)
+    st = srv.run()
+    srv.close()
+    log("serve: %s" % json.dumps(st))
+    return {"requests": st["requests"], "arrival_rate_per_s": 50.0,
+            "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
+            "throughput": st["throughput"], "unit": "bubble-steps/s", "wall_s": st["wall_s"],
+            "max_resident": st["max_resident"], "peak_hbm_bytes": st["peak_bytes"],
+            "hbm_budget_bytes": args.serve_budget_gib << 30, "round_steps": args.serve_round,
+            "precision": "mixed"}
+
+
+COPY_MIB = 256
+
+
+def device_copy(mib, reps):
+    """A plain device-to-device copy of known size (HIP's copy kernel): the
+    reference reading that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    hip.hipDeviceSynchronize()
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, n, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def leg_probe(ca, args, ip):
+    """Child of the traffic passes: a few launches of the headline workload's
+    kernels, then the reference copy."""
+    b = ca.Batch(ca.batch_workload(256, precision=ca.PREC_MIXED))
+    b.time_kernels(3, ip)
+    b.close()
+    device_copy(COPY_MIB, 3)
+
+
+def read_counters(folder, counter):
+    """{kernel name: [value per dispatch]} from rocprofv3's counter CSV."""
+    import csv
+    import glob
+
+    out = {}
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return out
+
+
+def measure_traffic(kernel):
+    """HBM bytes per launch of `kernel` over the headline workload, from two
+    rocprofv3 --pmc passes over a child run of this script (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch), each scaled by what the same counter reads
+    for the copy of COPY_MIB MiB.  Run before this process touches the GPU."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        folder = tempfile.mkdtemp(prefix="cubble_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", folder, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--legs", "probe",
+               "--no-cpu-baseline", "--no-traffic"]
+        log("traffic: " + " ".join(cmd))
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, timeout=90, capture_output=True)
+        except subprocess.TimeoutExpired:
+            return None, "rocprofv3 %s pass timed out" % counter
+        vals = read_counters(folder, counter)
+        shutil.rmtree(folder, ignore_errors=True)
+        copy = [v for k, v in vals.items() if "copy" in k.lower()]
+        mine = vals.get(kernel) or next((v for k, v in vals.items() if k.startswith(kernel + "(")),
+                                        None)
+        if r.returncode not in (0, -11, 139) and not mine:
+            return None, "rocprofv3 %s pass failed (rc %d)" % (counter, r.returncode)
+        if not copy or not mine:
+            return None, "%s pass: no reading for %s or the copy kernel" % (counter, kernel)
+        copy_kib = max(max(c) for c in copy)
+        scale = (COPY_MIB << 20) / (copy_kib * 1024.0)
+        res[counter] = {"kib_per_dispatch": sum(mine) / len(mine), "copy_kib": copy_kib,
+                        "scale": scale}
+    total = sum(v["kib_per_dispatch"] * 1024.0 * v["scale"] for v in res.values())
+    return total, res
+
+
+def cpu_baseline(cfgs, steps, budget_s):
+    """The OpenMP FP64 restatement on the host cores over the headline
+    workload's foams in order, `steps` steps each, until the time budget."""
     import oracle as ora
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     ora.set_threads(threads)
-    f = ora.Foam.synthetic(dim=cfg.dim, count=cfg.num_start, radius_dist=cfg.radius_dist,
-                           r_mean=cfg.r_mean, r_std=cfg.r_std, r_lo=cfg.r_lo, r_hi=cfg.r_hi,
-                           phi=cfg.phi, kappa=cfg.kappa, k_param=cfg.k_param, mu=cfg.mu,
-                           sigma=cfg.sigma, tol=cfg.error_tolerance, dt0=cfg.dt0,
-                           avg_rad=cfg.r_mean, seed=cfg.seed)
-    elapsed, steps = 0.0, 0
-    b0 = f.scalars()["bubble_steps"]
-    while elapsed < budget_s and steps < 500:
-        elapsed += f.run(2)
-        steps += 2
-    bs = f.scalars()["bubble_steps"] - b0
-    return {"value": bs / elapsed, "unit": "bubble-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d accepted steps of the same workload (%d bubbles) on the OpenMP FP64 "
-                      "oracle, %.1f s" % (steps, cfg.num_start, elapsed)}
+    elapsed, work, done, partial = 0.0, 0.0, 0, 0
+    for c in cfgs:
+        f = ora.Foam.synthetic(dim=c.dim, count=c.num_start, radius_dist=c.radius_dist,
+                               r_mean=c.r_mean, r_std=c.r_std, r_lo=c.r_lo, r_hi=c.r_hi,
+                               phi=c.phi, kappa=c.kappa, k_param=c.k_param, mu=c.mu,
+                               sigma=c.sigma, tol=c.error_tolerance, dt0=c.dt0,
+                               avg_rad=c.r_mean, seed=c.seed)
+        b0 = f.scalars()["bubble_steps"]
+        n = 0
+        while n < steps and elapsed < budget_s:
+            elapsed += f.run(10)
+            n += 10
+        work += f.scalars()["bubble_steps"] - b0
+        if n >= steps:
+            done += 1
+        else:
+            partial = n
+            break
+    return {"value": work / elapsed, "unit": "bubble-steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d foams of the 256-foam workload, %d steps each%s, FP64 OpenMP "
+                      "restatement, %.1f s" % (done, steps, " (+%d steps of the next)" % partial
+                                               if partial else "", elapsed)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="foam3d_1e6", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--steps", type=int, default=1000, help="timed steps of the batched workload")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--legs", default="batch,plain,foam,serve")
+    ap.add_argument("--foam-steps", type=int, default=2000)
+    ap.add_argument("--tune-steps", type=int, default=32)
+    ap.add_argument("--tuning-cache", default=None)
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shapes bubbles_per_thread,search_cells,graph_steps "
+                         "instead of autotuning")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--serve-round", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
     args = ap.parse_args()
@@ -89,66 +346,59 @@ def main():
         print("bench.py: this build measures one GPU only (the path has no multi-GPU sharding); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
 
     import cubble_amd as ca
 
-    desc, default_steps = WORKLOADS[args.workload]
-    steps = args.steps if args.steps is not None else default_steps
-    prec = ca.PREC_MIXED if args.precision == "mixed" else ca.PREC_FP64
-    cfg = ca.Config.workload(args.workload, precision=prec)
     ip = ca.IntegrationParams(True, False, True)
-
-    t_create = time.perf_counter()
-    sim = ca.Simulator(cfg)
-    t_create = time.perf_counter() - t_create
-    print("bench: created %s in %.2f s" % (args.workload, t_create), file=sys.stderr, flush=True)
-    if args.warmup > 0:
-        sim.run_steps(args.warmup, ip)
-    r0 = sim.scalars()
-    t0 = time.perf_counter()
-    r1 = sim.run_steps(steps, ip)
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    bubble_steps = r1["bubble_steps"] - r0["bubble_steps"]
-    value = bubble_steps / elapsed
-    print("bench: %d steps in %.3f s, %d attempts, %d searches, %d removals" %
-          (steps, elapsed, r1["attempts"] - r0["attempts"], r1["searches"] - r0["searches"],
-           r1["removals"] - r0["removals"]), file=sys.stderr, flush=True)
-
-    sc = sim.scalars()
-    n = sc["count"]
-    p = sc["num_pairs"] / max(n, 1)
-    kt = sim.time_kernels(args.kernel_reps, ip)
-    kb = kernel_bytes(cfg.dim, p)
-    times = {"predict": kt["predict_ms"], "pair": kt["pair_ms"], "correct": kt["correct_ms"]}
-    dom = max(times, key=times.get)
-    achieved = kb[dom] * n / (times[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_" + dom, "bytes_per_bubble": round(kb[dom], 2),
-                "launch_ms": round(times[dom], 4),
-                "kernels_ms": {k: round(v, 4) for k, v in times.items()},
-                "step_bytes_per_bubble": round(step_bytes(cfg.dim, p), 2),
-                "step_gbs": round(step_bytes(cfg.dim, p) * value / 1e9, 1)}
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "bubble-steps/s", "n_gpus": 1, "steps": steps,
-        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == ca.PREC_FP64 else "f64+f32",
-        "data": "synthetic (seeded; uniform positions, box sized for exact phi)",
-        "config": {"workload": args.workload, "description": desc, "bubbles": cfg.num_start,
-                   "dim": cfg.dim, "phi": cfg.phi, "precision": args.precision,
-                   "kappa": cfg.kappa, "K": cfg.k_param, "tol": cfg.error_tolerance,
-                   "pairs_per_bubble": round(p, 3), "final_count": n,
-                   "attempts": r1["attempts"] - r0["attempts"],
-                   "searches": r1["searches"] - r0["searches"],
-                   "removals": r1["removals"] - r0["removals"]},
-        "roofline": roofline,
-    }
-    sim.close()
+    if "probe" in legs:
+        leg_probe(ca, args, ip)
+        return
+    traffic = (None, "not measured")
+    if "batch" in legs and not args.no_traffic:
+        # counter passes first: this process has not touched the GPU yet
+        traffic = measure_traffic("k_pair_mixed")
+        log("traffic: %s" % json.dumps(traffic))
+    out = {"metric": METRIC, "unit": "bubble-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic (seeded; uniform positions, box sized for "
+                                        "exact phi; random radii)"}
+    cfgs = ca.batch_workload(256, precision=ca.PREC_MIXED)
+    if "batch" in legs:
+        b, cfgs = leg_batch(ca, args, ip)
+        out.update({"value": b["value"], "ms_per_step": b["elapsed_s"] / args.steps * 1e3,
+                    "dtype": "f64 state, f32 pair math",
+                    "config": {"workload": "batch256_mixed",
+                               "description": "256 independent foams, half 2D (normal radii) and "
+                                              "half 3D (log-normal radii), bubble counts "
+                                              "log-uniform in [1e3, 1e5], phi U[0.8, 0.95], "
+                                              "kappa logU[1e-3, 1e-1], K 0.1, mixed precision, "
+                                              "autotuned launch shapes, graph-replayed",
+                               "foams": 256, "bubbles_start": b["bubbles_start"],
+                               "bubbles_end": b["bubbles_end"],
+                               "pairs_per_bubble": b["pairs_per_bubble"],
+                               "tuning": b["tuning"]["chosen"],
+                               "autotune": {k: v for k, v in b["tuning"].items() if k != "chosen"},
+                               "counters": b["counters"]},
+                    "roofline": b["roofline"]})
+        rf = out["roofline"]
+        if rf["kernel"] == "k_pair_mixed" and traffic[0] is not None:
+            rf["traffic"] = round(traffic[0])
+            rf["traffic_counters"] = traffic[1]
+        else:
+            rf["traffic_note"] = traffic[1] if traffic[0] is None else \
+                "counters taken for k_pair_mixed, not the dominant kernel"
+    if "plain" in legs:
+        pl = leg_plain(ca, args, ip, cfgs)
+        out["plain_fp64_same_foams"] = pl
+        if "value" in out:
+            out["speedup_vs_plain_fp64"] = round(out["value"] / pl["value"], 3)
+    if "foam" in legs:
+        out["foam3d_1e6_fp64"] = leg_foam(ca, args, ip)
+    if "serve" in legs:
+        out["serving"] = leg_serve(ca, args, ip)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(cfgs, args.steps, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 

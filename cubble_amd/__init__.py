@@ -341,6 +341,44 @@ class Config:
         return cls(**base)
 
 
+def batch_workload(num_foams=256, seed=2024, precision=PREC_MIXED, lo=1e3, hi=1e5):
+    """The batched workload: independent foams, alternately 2D (normal radii,
+    sd 0.1, clipped to [0.5, 1.5]) and 3D (log-normal radii, sigma 0.15),
+    bubble counts log-uniform in [lo, hi], phi uniform in [0.8, 0.95], kappa
+    log-uniform in [1e-3, 1e-1], K = 0.1, each with its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(num_foams):
+        dim = 2 + (k % 2)
+        n = int(round(10 ** rng.uniform(np.log10(lo), np.log10(hi))))
+        phi = float(rng.uniform(0.8, 0.95))
+        kappa = float(10 ** rng.uniform(-3, -1))
+        dist = dict(radius_dist=0, r_std=0.1, r_lo=0.5, r_hi=1.5) if dim == 2 else \
+            dict(radius_dist=1, r_std=0.15)
+        out.append(Config(dim=dim, num_start=n, phi=phi, kappa=kappa, k_param=0.1, mu=1.0,
+                          error_tolerance=1e-4, dt0=1e-4, seed=seed * 1000 + k,
+                          precision=precision, **dist))
+    return out
+
+
+def serving_requests(num_requests, rate_per_s=50.0, seed=77, precision=PREC_MIXED, lo=1e3,
+                     hi=1e6):
+    """The serving workload: a Poisson stream of 3D foam requests, bubble counts
+    log-uniform in [lo, hi], log-normal radii (sigma 0.15), phi uniform in
+    [0.8, 0.95].  Returns [(config, arrival_s)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(num_requests):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        n = int(round(10 ** rng.uniform(np.log10(lo), np.log10(hi))))
+        phi = float(rng.uniform(0.8, 0.95))
+        out.append((Config(dim=3, num_start=n, radius_dist=1, r_std=0.15, phi=phi, kappa=0.01,
+                           k_param=0.1, mu=1.0, error_tolerance=1e-4, dt0=1e-4,
+                           seed=seed * 1000 + k, precision=precision), t))
+    return out
+
+
 def _dptr(a):
     return a.ctypes.data_as(_dp)
 

@@ -152,7 +152,71 @@ __device__ __forceinline__ void wrapped_diff(const Consts &c, double x1, double 
     for (int k = 0; k < D; ++k) {
         const double a = d1[k] < 0 ? -d1[k] : d1[k];
         const double t = c.interval[k] - a;
-        if (!c.wall[k] && t * t < d1[k] * d1[k]) d[k] = t * (d1[k] < 0 ? 1.0 : -1.0);
+        // selects, not branches: the same operations and rounding either way
+        const double w = d1[k] < 0 ? t : -t;
+        d[k] = (!c.wall[k] && t * t < d1[k] * d1[k]) ? w : d1[k];
+    }
+}
+
+// Lane l's double, broadcast to the wave (l must be wave-uniform).
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Neighbour gather of bubble i over its list (column-major, nbr[q * cap + i])
+// in chunks of kGather: the chunk's list entries are loaded, then all their
+// records, then `body(j, record)` runs for each in ascending list order.  A
+// lane's loads of one chunk are independent, so their latencies overlap
+// instead of adding up entry by entry.  Past the end of the list a lane
+// re-reads its own record and skips the body.
+constexpr int kGather = 8;
+
+// The buffers live in device global memory; pointers read from the foam
+// header are generic, so loads through them compile to flat loads, whose
+// completion waits also cover LDS traffic.  gmem() restates the address
+// space so they compile to global loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
+// For data a kernel only reads: the constant address space lets wave-uniform
+// loads go through the scalar cache into SGPRs.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *cmem(const T *p) {
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
+template <typename Body>
+__device__ __forceinline__ void for_neighbours(const int *__restrict__ nbr_,
+                                               const D4 *__restrict__ p4_, size_t cap, int i,
+                                               int m_nn, Body &&body) {
+    const auto nbr = gmem(nbr_);
+    const auto p4 = gmem(p4_);
+    for (int q0 = 0; q0 < m_nn; q0 += kGather) {
+        int js[kGather];
+        D4 pj[kGather];
+        // every load is issued (past the end: the last entry, in bounds), then
+        // the entries past the end are masked off
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            const int q = q0 + u < m_nn ? q0 + u : m_nn - 1;
+            const int j = nbr[(size_t)q * cap + i];
+            js[u] = q0 + u < m_nn ? j : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            const int j = js[u] >= 0 ? js[u] : i;
+            pj[u] = D4{p4[j].x, p4[j].y, p4[j].z, p4[j].r};
+        }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+            if (js[u] >= 0) body(js[u], pj[u]);
     }
 }
 

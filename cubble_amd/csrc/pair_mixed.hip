@@ -42,9 +42,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
         float acc[3] = {0.f, 0.f, 0.f};
         float ovl = 0.f, dr = 0.f;
         const int m_nn = nnA[i];
-        for (int q = 0; q < m_nn; ++q) {
-            const int j = nbr[(size_t)q * cap + i];
-            const D4 pj = p4[j];
+        for_neighbours(nbr, p4, cap, i, m_nn, [&](int j, const D4 &pj) {
             const double rj = pj.r;
             double dd[3] = {xi - pj.x, yi - pj.y, D == 3 ? zi - pj.z : 0.0};
 #pragma unroll
@@ -78,7 +76,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
                     dr += oa * (__frcp_rn(rj_f) - inv_ri);
                 }
             }
-        }
+        });
 #pragma unroll
         for (int a = 0; a < D; ++a) R.vp[a][i] = (double)acc[a];
         R.vp[3][i] = (double)dr;

@@ -215,106 +215,253 @@ __device__ void swap_reordered_roles(Roles &W) {
 
 // Sorted, de-duplicated neighbour coordinates of cell coordinate `x` along
 // one axis: periodic axes wrap, wall axes drop out-of-range cells.  With
-// fewer than 3 cells the wrapped offsets coincide and appear once.
-__device__ __forceinline__ int axis_cells(int x, int n, bool wall, int out[3]) {
-    int a = x - 1, b = x, c = x + 1;
-    int cnt = 0;
+// fewer than 3 cells the wrapped offsets coincide and appear once.  Held in
+// three named values, not an array, so no kernel indexes private memory.
+struct AxisCells {
+    int v0, v1, v2, n;
+    __device__ __forceinline__ int at(int k) const { return 0 == k ? v0 : 1 == k ? v1 : v2; }
+};
+
+__device__ __forceinline__ AxisCells axis_cells(int x, int n, bool wall) {
+    AxisCells r;
     if (wall) {
-        if (a >= 0) out[cnt++] = a;
-        out[cnt++] = b;
-        if (c < n) out[cnt++] = c;
-        return cnt;
+        const bool lo = x - 1 >= 0, hi = x + 1 < n;
+        r.v0 = lo ? x - 1 : x;
+        r.v1 = lo ? x : x + 1;
+        r.v2 = x + 1;
+        r.n = 1 + (lo ? 1 : 0) + (hi ? 1 : 0);
+        return r;
     }
-    a = (a + n) % n;
-    c = c % n;
-    // sort three values
-    int v0 = a, v1 = b, v2 = c, t;
+    int v0 = (x - 1 + n) % n, v1 = x, v2 = (x + 1) % n, t;
     if (v0 > v1) { t = v0; v0 = v1; v1 = t; }
     if (v1 > v2) { t = v1; v1 = v2; v2 = t; }
     if (v0 > v1) { t = v0; v0 = v1; v1 = t; }
-    out[cnt++] = v0;
-    if (v1 != v0) out[cnt++] = v1;
-    if (v2 != v1) out[cnt++] = v2;
-    return cnt;
+    // drop duplicates, keeping order
+    const bool d1 = v1 == v0, d2 = v2 == v1;
+    r.v0 = v0;
+    r.v1 = d1 ? v2 : v1;
+    r.v2 = v2;
+    r.n = 3 - (d1 ? 1 : 0) - (d2 ? 1 : 0);
+    return r;
 }
 
-// Neighbour lists, one wavefront per cell.  The linear cell id
-// z*cx*cy + y*cx + x grows lexicographically in (z, y, x), so walking the
-// sorted per-axis cell lists in that nesting visits cells -- and, since
-// bubbles are stored in cell order, candidates -- in ascending index order;
-// consecutive x-cells of one row form one contiguous run of bubbles.  The 64
-// lanes take 64 consecutive candidates of a run (one coalesced 32-byte
-// record each); the cell's own bubbles are taken one at a time as uniform
-// (scalar) loads; hits are found with a wave ballot and appended to that
-// bubble's list in lane order, which keeps every list ascending.  Lists
-// longer than the capacity are counted, never written out of bounds.
+// The x-cells of one row as at most two runs of consecutive cells,
+// [a0, b0] and (if two) [a1, b1], in ascending order.
+struct XRuns {
+    int a0, b0, a1, b1;
+    bool two;
+};
+
+__device__ __forceinline__ XRuns x_runs(const AxisCells &X) {
+    int f0 = 0;
+    if (X.n > 1 && X.v1 == X.v0 + 1) f0 = 1;
+    if (1 == f0 && X.n > 2 && X.v2 == X.v1 + 1) f0 = 2;
+    XRuns r;
+    r.a0 = X.v0;
+    r.b0 = X.at(f0);
+    r.two = f0 + 1 < X.n;
+    r.a1 = r.two ? X.at(f0 + 1) : 0;
+    r.b1 = r.two ? X.at(X.n - 1) : -1;
+    return r;
+}
+
+// Wave-uniform append of the ballot `m` of candidates j to own bubble i's
+// list (lane `ii` of the group keeps the count).
+__device__ __forceinline__ void append_hits(unsigned long long m, bool hit, int ii, int i, int j,
+                                            unsigned long long lt, int *__restrict__ nbr,
+                                            size_t cap, int nbr_cap, int &cnt) {
+    const int base = __shfl(cnt, ii);
+    if (hit) {
+        const int pos = base + __popcll(m & lt);
+        if (pos < nbr_cap) nbr[(size_t)pos * cap + i] = j;
+    }
+    if ((int)(threadIdx.x & 63) == ii) cnt += __popcll(m);
+}
+
+__device__ __forceinline__ void finish_group(int cb, int nb, int cnt, int nbr_cap,
+                                             int *__restrict__ nn, int &sum_nn, int &max_nn,
+                                             int &ovf) {
+    if ((int)(threadIdx.x & 63) < nb) {
+        nn[cb + (int)(threadIdx.x & 63)] = cnt < nbr_cap ? cnt : nbr_cap;
+        sum_nn += cnt;
+        max_nn = max(max_nn, cnt);
+        if (cnt > nbr_cap) ovf = 1;
+    }
+}
+
+// Neighbour lists, one wavefront per cell, any box (the general path).
+// The linear cell id z*cx*cy + y*cx + x grows lexicographically in
+// (z, y, x), so walking the sorted per-axis cell lists in that nesting
+// visits cells -- and, since bubbles are stored in cell order, candidates --
+// in ascending index order; consecutive x-cells of one row form one
+// contiguous run of bubbles.  The 64 lanes take 64 consecutive candidates of
+// a run (one coalesced 32-byte record each); the cell's own bubbles are
+// wave-uniform scalar loads; hits are found with a wave ballot and appended
+// to that bubble's list in lane order, which keeps every list ascending.
+// Lists longer than the capacity are counted, never written out of bounds.
+// `cell` must be wave-uniform.
 template <int D>
-__device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4,
-                                            const int *__restrict__ start, int *__restrict__ nbr,
-                                            int *__restrict__ nn, const int cd[3], size_t cap,
-                                            int nbr_cap, int cell, int &sum_nn, int &max_nn,
-                                            int &ovf) {
+__device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *__restrict__ x4_,
+                                                    const int *__restrict__ start_,
+                                                    int *__restrict__ nbr, int *__restrict__ nn,
+                                                    const int cd[3], size_t cap, int nbr_cap,
+                                                    int cell, int &sum_nn, int &max_nn,
+                                                    int &ovf) {
+    const auto x4 = gmem(x4_);
+    const auto x4s = cmem(x4_);
+    const auto start = cmem(start_);
     const int lane = (int)(threadIdx.x & 63);
+    cell = uniform_i(cell);
     const int i0 = start[cell], i1 = start[cell + 1];
-    if (i0 >= i1) return;  // uniform per wave
+    if (i0 >= i1) return;
     const int ccx = cell % cd[0];
     const int ccy = (cell / cd[0]) % cd[1];
     const int ccz = cell / (cd[0] * cd[1]);
-    int xs[3], ys[3], zs[3];
-    const int nx = axis_cells(ccx, cd[0], c.wall[0] != 0, xs);
-    const int ny = axis_cells(ccy, cd[1], c.wall[1] != 0, ys);
-    int nz = 1;
-    zs[0] = 0;
-    if (3 == D) nz = axis_cells(ccz, cd[2], c.wall[2] != 0, zs);
+    const AxisCells X = axis_cells(ccx, cd[0], c.wall[0] != 0);
+    const AxisCells Y = axis_cells(ccy, cd[1], c.wall[1] != 0);
+    const AxisCells Z = 3 == D ? axis_cells(ccz, cd[2], c.wall[2] != 0) : AxisCells{0, 0, 0, 1};
+    const XRuns xr = x_runs(X);
     const double skin = c.skin;
     const unsigned long long lt = (1ull << lane) - 1ull;
     for (int cb = i0; cb < i1; cb += 64) {
         const int nb = min(64, i1 - cb);
         int cnt = 0;
-        for (int a = 0; a < nz; ++a) {
-            for (int b = 0; b < ny; ++b) {
-                const int row = zs[a] * cd[0] * cd[1] + ys[b] * cd[0];
-                int e = 0;
-                while (e < nx) {
-                    int f = e;
-                    while (f + 1 < nx && xs[f + 1] == xs[f] + 1) ++f;
-                    const int j0 = start[row + xs[e]], j1 = start[row + xs[f] + 1];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+#pragma unroll
+                for (int part = 0; part < 2; ++part) {
+                    if (a >= Z.n || b >= Y.n || (1 == part && !xr.two)) continue;
+                    const int row = Z.at(a) * cd[0] * cd[1] + Y.at(b) * cd[0];
+                    const int j0 = start[row + (0 == part ? xr.a0 : xr.a1)];
+                    const int j1 = start[row + (0 == part ? xr.b0 : xr.b1) + 1];
                     for (int jb = j0; jb < j1; jb += 64) {
                         const int j = jb + lane;
                         const bool valid = j < j1;
-                        D4 pj = D4{0.0, 0.0, 0.0, 0.0};
-                        if (valid) pj = x4[j];
+                        const int jl = valid ? j : j0;
+                        const D4 pj = D4{x4[jl].x, x4[jl].y, 3 == D ? x4[jl].z : 0.0, x4[jl].r};
                         for (int ii = 0; ii < nb; ++ii) {
                             const int i = cb + ii;
-                            const D4 pi = x4[i];
-                            bool hit = false;
-                            if (valid && j != i) {
-                                const double maxd = pi.r + pj.r + skin;
-                                double d[3];
-                                wrapped_diff<D>(c, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, d);
-                                hit = sq_len(d) < maxd * maxd;
-                            }
+                            const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
+                            const double pz = 3 == D ? x4s[i].z : 0.0;
+                            const double maxd = pr + pj.r + skin;
+                            double d[3];
+                            wrapped_diff<D>(c, px, py, pz, pj.x, pj.y, pj.z, d);
+                            const bool hit = valid && j != i && sq_len(d) < maxd * maxd;
                             const unsigned long long m = __ballot(hit);
-                            if (m) {
-                                const int base = __shfl(cnt, ii);
-                                if (hit) {
-                                    const int pos = base + __popcll(m & lt);
-                                    if (pos < nbr_cap) nbr[(size_t)pos * cap + i] = j;
-                                }
-                                if (lane == ii) cnt += __popcll(m);
-                            }
+                            if (m) append_hits(m, hit, ii, i, j, lt, nbr, cap, nbr_cap, cnt);
                         }
                     }
-                    e = f + 1;
                 }
             }
         }
-        if (lane < nb) {
-            nn[cb + lane] = cnt < nbr_cap ? cnt : nbr_cap;
-            sum_nn += cnt;
-            max_nn = max(max_nn, cnt);
-            if (cnt > nbr_cap) ovf = 1;
+        finish_group(cb, nb, cnt, nbr_cap, nn, sum_nn, max_nn, ovf);
+    }
+}
+
+constexpr int kMaxRuns = 18;  // 3 z-rows x 3 y-rows x 2 x-runs
+
+// Neighbour lists of one cell when every periodic axis has at least 5 cells
+// (the common case).  The candidate runs of the cell (at most two per row:
+// the x-cells of a row split only where the row wraps) are concatenated, in
+// ascending bubble order, into one sequence that the 64 lanes walk 64 at a
+// time, so partly filled chunks occur once per cell instead of once per run.
+// With 5 or more cells per axis the minimum image is decided by the cells
+// alone: a candidate in a neighbouring cell that does not cross the boundary
+// is within 2/5 of the box and needs no wrap (|d| < L - |d|), one across the
+// boundary is at least 3/5 away and always takes the image L - |d| -- the
+// same operations as the general minimum-image difference, applied without
+// its test.  Same order, same lists, same rounding as search_cell_general.
+// `cell` must be wave-uniform.
+template <int D>
+__device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4_,
+                                            const int *__restrict__ start_,
+                                            int *__restrict__ nbr, int *__restrict__ nn,
+                                            const int cd[3], size_t cap, int nbr_cap, int cell,
+                                            int &sum_nn, int &max_nn, int &ovf) {
+    const auto x4 = gmem(x4_);
+    const auto x4s = cmem(x4_);
+    const auto start = cmem(start_);
+    const int lane = (int)(threadIdx.x & 63);
+    cell = uniform_i(cell);
+    const int i0 = start[cell], i1 = start[cell + 1];
+    if (i0 >= i1) return;
+    const int ccx = cell % cd[0];
+    const int ccy = (cell / cd[0]) % cd[1];
+    const int ccz = cell / (cd[0] * cd[1]);
+    const AxisCells X = axis_cells(ccx, cd[0], c.wall[0] != 0);
+    const AxisCells Y = axis_cells(ccy, cd[1], c.wall[1] != 0);
+    const AxisCells Z = 3 == D ? axis_cells(ccz, cd[2], c.wall[2] != 0) : AxisCells{0, 0, 0, 1};
+    const XRuns xr = x_runs(X);
+    auto far = [](int a, int b) { return a - b > 1 || b - a > 1; };
+    const int fx0 = far(xr.a0, ccx) ? 1 : 0, fx1 = far(xr.a1, ccx) ? 1 : 0;
+    int j0[kMaxRuns], len[kMaxRuns], pre[kMaxRuns], fl[kMaxRuns];
+    int total = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const bool row_ok = a < Z.n && b < Y.n;
+            const int zc = Z.at(a), yc = Y.at(b);
+            const int row = zc * cd[0] * cd[1] + yc * cd[0];
+            const int fyz = (far(yc, ccy) ? 2 : 0) | (3 == D && far(zc, ccz) ? 4 : 0);
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+                const int s = (a * 3 + b) * 2 + part;
+                const bool ok = row_ok && (0 == part || xr.two);
+                const int js = ok ? start[row + (0 == part ? xr.a0 : xr.a1)] : 0;
+                const int je = ok ? start[row + (0 == part ? xr.b0 : xr.b1) + 1] : 0;
+                j0[s] = js;
+                len[s] = je - js;
+                pre[s] = total;
+                fl[s] = fyz | (0 == part ? fx0 : fx1);
+                total += je - js;
+            }
         }
+    }
+    const double skin = c.skin;
+    const double L[3] = {c.interval[0], c.interval[1], c.interval[2]};
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int cb = i0; cb < i1; cb += 64) {
+        const int nb = min(64, i1 - cb);
+        int cnt = 0;
+        for (int p0 = 0; p0 < total; p0 += 64) {
+            const int p = p0 + lane;
+            int j = -1, f = 0;
+#pragma unroll
+            for (int s = 0; s < kMaxRuns; ++s) {
+                const int o = p - pre[s];
+                if (o >= 0 && o < len[s]) {
+                    j = j0[s] + o;
+                    f = fl[s];
+                }
+            }
+            const bool valid = j >= 0;
+            const int jl = valid ? j : i0;
+            const D4 pj = D4{x4[jl].x, x4[jl].y, 3 == D ? x4[jl].z : 0.0, x4[jl].r};
+            const bool anyflip = __ballot(valid && f != 0) != 0ull;
+            for (int ii = 0; ii < nb; ++ii) {
+                const int i = cb + ii;
+                const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
+                const double pz = 3 == D ? x4s[i].z : 0.0;
+                double d[3] = {px - pj.x, py - pj.y, 3 == D ? pz - pj.z : 0.0};
+                if (anyflip) {
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const double a = d[k] < 0 ? -d[k] : d[k];
+                        const double t = L[k] - a;
+                        const double w = d[k] < 0 ? t : -t;
+                        d[k] = ((f >> k) & 1) ? w : d[k];
+                    }
+                }
+                const double maxd = pr + pj.r + skin;
+                const bool hit = valid && j != i && sq_len(d) < maxd * maxd;
+                const unsigned long long m = __ballot(hit);
+                if (m) append_hits(m, hit, ii, i, j, lt, nbr, cap, nbr_cap, cnt);
+            }
+        }
+        finish_group(cb, nb, cnt, nbr_cap, nn, sum_nn, max_nn, ovf);
     }
 }
 
@@ -332,13 +479,27 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
     const int end = min(base + F->scells, ncell);
     const size_t cap = (size_t)F->cap;
     int sum_nn = 0, max_nn = 0, ovf = 0;
+    // the fast path needs 5 or more cells along every periodic axis
+    bool fast = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k < c.dim && !c.wall[k] && cd[k] < 5) fast = false;
     for (int cell = base + (int)(threadIdx.x >> 6); cell < end; cell += kWaves) {
-        if (3 == c.dim)
-            search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell, sum_nn,
-                           max_nn, ovf);
-        else
-            search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell, sum_nn,
-                           max_nn, ovf);
+        if (3 == c.dim) {
+            if (fast)
+                search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
+                               sum_nn, max_nn, ovf);
+            else
+                search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap,
+                                       cell, sum_nn, max_nn, ovf);
+        } else {
+            if (fast)
+                search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
+                               sum_nn, max_nn, ovf);
+            else
+                search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap,
+                                       cell, sum_nn, max_nn, ovf);
+        }
     }
     sum_nn = block_sum_i(sum_nn);
     max_nn = block_max_i(max_nn);

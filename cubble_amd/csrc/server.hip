@@ -229,6 +229,8 @@ int cubble_server_run(cubble_server *s, cubble_server_stats *stats) {
         max_resident = std::max(max_resident, (int)s->resident.size());
         peak = std::max(peak, (double)e.arena.in_use());
         if (s->resident.empty()) {
+            // admission may have failed the last queued request
+            if (s->queue.empty()) break;
             // idle until the next arrival
             const double wait = s->reqs[s->queue.front()].r.arrival_s - now_s(t0);
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));

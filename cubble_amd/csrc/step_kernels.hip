@@ -90,9 +90,7 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
         double acc[3] = {0.0, 0.0, 0.0};
         double ovl = 0.0, dr = 0.0;
         const int m_nn = nnA[i];
-        for (int q = 0; q < m_nn; ++q) {
-            const int j = nbr[(size_t)q * cap + i];
-            const D4 pj = p4[j];
+        for_neighbours(nbr, p4, cap, i, m_nn, [&](int j, const D4 &pj) {
             const double rj = pj.r;
             const double r1 = i < j ? ri : rj;
             const double r2 = i < j ? rj : ri;
@@ -126,7 +124,7 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
                     dr += oa * (1.0 / rj - inv_ri);
                 }
             }
-        }
+        });
 #pragma unroll
         for (int a = 0; a < D; ++a) R.vp[a][i] = acc[a];
         R.vp[3][i] = dr;

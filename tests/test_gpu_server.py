@@ -28,18 +28,23 @@ def alone(cfg, steps):
 
 def test_server_results_equal_alone_and_overlap():
     cfgs = request_configs()
+    # the first request runs long; the others arrive while it is on the GPU
+    steps = [1500] + [45] * (len(cfgs) - 1)
     srv = ca.Server(hbm_budget_bytes=1 << 30, round_steps=20, keep_states=True)
-    ids = [srv.submit(c, max_steps=45, arrival_s=0.03 * k) for k, c in enumerate(cfgs)]
+    ids = [srv.submit(c, max_steps=m, arrival_s=0.002 * k)
+           for k, (c, m) in enumerate(zip(cfgs, steps))]
     stats = srv.run()
     assert stats["requests"] == len(cfgs)
-    assert stats["max_resident"] >= 2          # requests overlapped on the device
     assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
-    for rid, cfg in zip(ids, cfgs):
+    first = srv.result(ids[0])
+    overlapped = [srv.result(i) for i in ids[1:] if srv.result(i)["admit_s"] < first["finish_s"]]
+    assert overlapped and stats["max_resident"] >= 2   # admitted while another ran
+    for rid, cfg, m in zip(ids, cfgs, steps):
         r = srv.result(rid)
-        assert r["status"] == 2 and r["steps"] == 45
+        assert r["status"] == 2 and r["steps"] == m
         assert r["admit_s"] >= r["arrival_s"] and r["finish_s"] > r["admit_s"]
         gs, gsc = srv.get_state(rid)
-        ws, wsc = alone(cfg, 45)
+        ws, wsc = alone(cfg, m)
         for key in gs:
             assert np.array_equal(gs[key], ws[key]), key
         assert gsc["dt"] == wsc["dt"] and gsc["bubble_steps"] == wsc["bubble_steps"]
@@ -47,16 +52,21 @@ def test_server_results_equal_alone_and_overlap():
 
 
 def test_server_admission_by_footprint():
-    cfgs = request_configs()[:4]
+    # four foams of one size: the budget holds two of them at a time
+    cfgs = [config(3, 3000, phi=0.85, seed=40 + k) for k in range(4)]
     need = max(ca.foam_footprint(c) for c in cfgs)
-    # room for about two foams at once
+    assert min(ca.foam_footprint(c) for c in cfgs) * 3 > 2.6 * need
     srv = ca.Server(hbm_budget_bytes=int(2.6 * need), round_steps=15)
-    for c in cfgs:
-        srv.submit(c, max_steps=30, arrival_s=0.0)
+    ids = [srv.submit(c, max_steps=30, arrival_s=0.0) for c in cfgs]
     stats = srv.run()
     assert stats["requests"] == 4
-    assert stats["max_resident"] <= 2
+    assert stats["max_resident"] == 2
     assert stats["peak_bytes"] <= 2.6 * need
+    res = [srv.result(i) for i in ids]
+    assert all(r["status"] == 2 and r["steps"] == 30 for r in res)
+    # the last two waited for memory: admitted only after an earlier one finished
+    first_done = min(r["finish_s"] for r in res[:2])
+    assert all(r["admit_s"] >= first_done for r in res[2:])
     srv.close()
 
 
