@@ -68,6 +68,7 @@ size_t carve_foam(Foam &F, char *base) {
     S.x4 = L.take<D4>(cap);
     S.ov = L.take<double>(cap);
     S.del = L.take<uint8_t>(cap);
+    S.dirty = L.take<uint8_t>(cap);
     S.nn = L.take<int32_t>(cap);
     S.cell_of = L.take<int32_t>(cap);
     S.members = L.take<int32_t>(cap);

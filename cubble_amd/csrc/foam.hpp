@@ -58,6 +58,7 @@ struct Scratch {
     D4 *x4;              // current x, y, z, r in cell order, written by the search
     double *ov;          // per-bubble overlap area sum
     uint8_t *del;        // deletion mark of the last attempt
+    uint8_t *dirty;      // list needs filtering after a removal (all zero otherwise)
     int32_t *nn;         // neighbour count
     int32_t *nbr;        // neighbour lists, column-major [nbr_cap][cap]
     int32_t *cell_of;    // cell of each bubble (old order)
@@ -65,8 +66,8 @@ struct Scratch {
     int32_t *cell_start; // num_cells + 1
     int32_t *cell_cursor;
     int32_t *members;    // unsorted cell segments
-    double *part;        // bubble-space block partials [kMaxBlocks][kPartD]
-    int32_t *part_i;     // bubble-space block partials [kMaxBlocks][kPartI]
+    double *part;        // bubble-space block partials [nblk][kPartD]
+    int32_t *part_i;     // bubble-space block partials [nblk][kPartI]
     int32_t *cpart;      // cell-space block partials (cell-count sums)
     int32_t *cpart_i;    // neighbour-search block partials [sblk][kPartI]
     int32_t *del_list;   // sorted deleted indices

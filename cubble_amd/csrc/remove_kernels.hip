@@ -8,34 +8,36 @@
 //                back (reference swapDataCountPairs, src/kernels.cu:775-857) --
 //                and the volume-fix multiplier M / S_V^(1/D)
 //                (src/kernels.cu:861-867)
-//   k_rm_apply : every slot below the new count pulls its state (a hole from
-//                its tail survivor), drops neighbours that were deleted, maps
-//                moved neighbours to their new slots, keeps the list
-//                ascending, and applies the volume fix (reference
+//   k_rm_mark  : marks the lists that change: those of the neighbours of every
+//                deleted bubble and of every moving tail survivor
+//   k_rm_apply : every slot below the new count applies the volume fix; a hole
+//                pulls its tail survivor's state and list, a marked bubble
+//                filters its own list -- deleted neighbours dropped, moved
+//                ones mapped to their new slots, kept ascending (reference
 //                addVolumeFixPairs, src/kernels.cu:859-936)
 //   k_rm_fin   : one block per foam commits the new count
 //
 // The reference appends the deletion list with an atomic counter, so its
 // hole / survivor pairing depends on thread timing; a sorted list makes the
 // pairing deterministic.  Its race-prone pair swap-from-the-back
-// (src/kernels.cu:873-909) is replaced by per-bubble list filtering.
+// (src/kernels.cu:873-909) is replaced by per-bubble list filtering, done
+// only for the few lists a removal touches.
 #include "dev_util.hpp"
 #include "launch.hpp"
 
 namespace cubble_amd {
 
-__global__ __launch_bounds__(kBlock) void k_rm_emit(Foam *foams, const int2 *bt) {
-    const Ctx cx = bubble_ctx(foams, bt);
-    Foam *F = cx.F;
-    if (!F->d.need_remove) return;
+namespace {
+
+__device__ __forceinline__ void emit_block(Foam *F, int lb) {
     const Scratch S = F->s;
     const int n = F->d.count, ipt = F->ipt;
-    const int cnt = S.part_i[cx.lb * kPartI + 0];
+    const int cnt = S.part_i[lb * kPartI + 0];
     if (cnt == 0) return;  // uniform per block
     int acc = 0;
-    for (int b = threadIdx.x; b < cx.lb; b += kBlock) acc += S.part_i[b * kPartI + 0];
+    for (int b = threadIdx.x; b < lb; b += kBlock) acc += S.part_i[b * kPartI + 0];
     int off = block_sum_i(acc);
-    const int base = cx.lb * kBlock * ipt;
+    const int base = lb * kBlock * ipt;
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         const int f = (i < n && S.del[i]) ? 1 : 0;
@@ -44,6 +46,109 @@ __global__ __launch_bounds__(kBlock) void k_rm_emit(Foam *foams, const int2 *bt)
         if (f) S.del_list[off + pre] = i;
         off += tot;
     }
+}
+
+// Items of the mark kernel: the deleted bubbles, then the moving survivors
+// (at most the old count, so the bubble-space blocks cover them).
+__device__ __forceinline__ void mark_block(Foam *F, int lb) {
+    const Scratch S = F->s;
+    const int nd = F->d.n_del, items = nd + F->d.n_holes;
+    const int nnew = F->d.count - nd;
+    const size_t cap = (size_t)F->cap;
+    const int base = lb * kBlock * F->ipt;
+    if (base >= items) return;  // uniform per block
+    for (int k = 0; k < F->ipt; ++k) {
+        const int t = base + k * kBlock + (int)threadIdx.x;
+        if (t >= items) break;
+        const int s = t < nd ? S.del_list[t] : S.plan_src[t - nd];
+        const int m = S.nn[s];
+        for (int q = 0; q < m; ++q) {
+            const int j = S.nbr[(size_t)q * cap + s];
+            if (j < nnew) S.dirty[j] = 1;  // same value from every writer
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, size_t cap, int base,
+                                           int ipt, int nnew, int h, double vm) {
+    int sum_nn = 0;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= nnew) break;
+        const bool hole = S.del[i] != 0;
+        const bool marked = S.dirty[i] != 0;
+        int src = i;
+        if (hole) {
+            int lo = 0, hi = h;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (S.del_list[mid] < i) lo = mid + 1;
+                else hi = mid;
+            }
+            src = S.plan_src[lo];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (2 == q && 2 == D) continue;
+                R.x[q][i] = R.x[q][src];
+                R.v[q][i] = R.v[q][src];
+                R.vo[q][i] = R.vo[q][src];
+                R.saved[q][i] = R.saved[q][src];
+            }
+            R.path[i] = R.path[src];
+            R.err[i] = R.err[src];
+#pragma unroll
+            for (int q = 0; q < D; ++q) R.wc[q][i] = R.wc[q][src];
+            R.idx[i] = R.idx[src];
+        }
+        int out = S.nn[src];
+        if (hole || marked) {
+            const int m = out;
+            out = 0;
+            for (int q = 0; q < m; ++q) {
+                const int j = S.nbr[(size_t)q * cap + src];
+                if (S.del[j]) continue;
+                const int jj = j >= nnew ? S.moved_to[j - nnew] : j;
+                // insertion keeps the list ascending (in place when src == i:
+                // writes stay at positions <= q)
+                int pos = out;
+                while (pos > 0) {
+                    const int prev = S.nbr[(size_t)(pos - 1) * cap + i];
+                    if (prev <= jj) break;
+                    S.nbr[(size_t)pos * cap + i] = prev;
+                    --pos;
+                }
+                S.nbr[(size_t)pos * cap + i] = jj;
+                ++out;
+            }
+            S.nn[i] = out;
+            if (marked) S.dirty[i] = 0;
+        }
+        R.x[3][i] = R.x[3][i] * vm;
+        sum_nn += out;
+    }
+    return sum_nn;
+}
+
+__device__ __forceinline__ void apply_block(Foam *F, int lb) {
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const int n = F->d.count, ipt = F->ipt;
+    const int nnew = n - F->d.n_del, h = F->d.n_holes;
+    const double vm = F->d.vol_fix;
+    const size_t cap = (size_t)F->cap;
+    const int base = lb * kBlock * ipt;
+    int sum_nn = 3 == F->c.dim ? apply_range<3>(R, S, cap, base, ipt, nnew, h, vm)
+                               : apply_range<2>(R, S, cap, base, ipt, nnew, h, vm);
+    sum_nn = block_sum_i(sum_nn);
+    if (threadIdx.x == 0) S.part_i[lb * kPartI + 1] = sum_nn;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_rm_emit(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    if (cx.F->d.need_remove) emit_block(cx.F, cx.lb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rm_plan(Foam *foams) {
@@ -75,74 +180,14 @@ __global__ __launch_bounds__(kBlock) void k_rm_plan(Foam *foams) {
     }
 }
 
-template <int D>
-__device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, size_t cap, int base,
-                                           int ipt, int nnew, int h, double vm) {
-    int sum_nn = 0;
-    for (int k = 0; k < ipt; ++k) {
-        const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= nnew) break;
-        int lo = 0, hi = h;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (S.del_list[mid] < i) lo = mid + 1;
-            else hi = mid;
-        }
-        const int src = (lo < h && S.del_list[lo] == i) ? S.plan_src[lo] : i;
-        if (src != i) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (2 == q && 2 == D) continue;
-                R.x[q][i] = R.x[q][src];
-                R.v[q][i] = R.v[q][src];
-                R.vo[q][i] = R.vo[q][src];
-                R.saved[q][i] = R.saved[q][src];
-            }
-            R.path[i] = R.path[src];
-            R.err[i] = R.err[src];
-#pragma unroll
-            for (int q = 0; q < D; ++q) R.wc[q][i] = R.wc[q][src];
-            R.idx[i] = R.idx[src];
-        }
-        const int m = S.nn[src];
-        int out = 0;
-        for (int q = 0; q < m; ++q) {
-            const int j = S.nbr[(size_t)q * cap + src];
-            if (S.del[j]) continue;
-            const int jj = j >= nnew ? S.moved_to[j - nnew] : j;
-            // insertion keeps the list ascending
-            int pos = out;
-            while (pos > 0) {
-                const int prev = S.nbr[(size_t)(pos - 1) * cap + i];
-                if (prev <= jj) break;
-                S.nbr[(size_t)pos * cap + i] = prev;
-                --pos;
-            }
-            S.nbr[(size_t)pos * cap + i] = jj;
-            ++out;
-        }
-        S.nn[i] = out;
-        R.x[3][i] = R.x[3][i] * vm;
-        sum_nn += out;
-    }
-    return sum_nn;
+__global__ __launch_bounds__(kBlock) void k_rm_mark(Foam *foams, const int2 *bt) {
+    const Ctx cx = cell_ctx(foams, bt);  // items in plain block order
+    if (cx.F->d.need_remove) mark_block(cx.F, cx.lb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rm_apply(Foam *foams, const int2 *bt) {
     const Ctx cx = bubble_ctx(foams, bt);
-    Foam *F = cx.F;
-    if (!F->d.need_remove) return;
-    const Roles R = F->r;
-    const Scratch S = F->s;
-    const int n = F->d.count, ipt = F->ipt;
-    const int nnew = n - F->d.n_del, h = F->d.n_holes;
-    const double vm = F->d.vol_fix;
-    const size_t cap = (size_t)F->cap;
-    const int base = cx.lb * kBlock * ipt;
-    int sum_nn = 3 == F->c.dim ? apply_range<3>(R, S, cap, base, ipt, nnew, h, vm)
-                               : apply_range<2>(R, S, cap, base, ipt, nnew, h, vm);
-    sum_nn = block_sum_i(sum_nn);
-    if (threadIdx.x == 0) S.part_i[cx.lb * kPartI + 1] = sum_nn;
+    if (cx.F->d.need_remove) apply_block(cx.F, cx.lb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rm_fin(Foam *foams) {
@@ -164,6 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_rm_fin(Foam *foams) {
 void launch_remove(const Grid &g) {
     hipLaunchKernelGGL(k_rm_emit, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_rm_plan, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
+    hipLaunchKernelGGL(k_rm_mark, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_rm_apply, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_rm_fin, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
 }
