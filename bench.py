@@ -225,12 +225,16 @@ def device_copy(mib, reps):
     hip.hipFree(b)
 
 
-def leg_probe(ca, args, ip):
-    """Child of the traffic passes: a few launches of the headline workload's
-    kernels, then the reference copy."""
-    b = ca.Batch(ca.batch_workload(256, precision=ca.PREC_MIXED))
-    b.time_kernels(3, ip)
-    b.close()
+def leg_probe(ca, args, ip, foam=False):
+    """Child of the counter passes: a few launches of the attempt kernels of
+    the headline workload (or, with foam=True, of the 1e6-bubble FP64 foam),
+    then the reference copy."""
+    if foam:
+        h = ca.Simulator(ca.Config.workload("foam3d_1e6"))
+    else:
+        h = ca.Batch(ca.batch_workload(256, precision=ca.PREC_MIXED))
+    h.time_kernels(3, ip)
+    h.close()
     device_copy(COPY_MIB, 3)
 
 
@@ -351,8 +355,8 @@ def main():
     import cubble_amd as ca
 
     ip = ca.IntegrationParams(True, False, True)
-    if "probe" in legs:
-        leg_probe(ca, args, ip)
+    if "probe" in legs or "probe-foam" in legs:
+        leg_probe(ca, args, ip, foam="probe-foam" in legs)
         return
     traffic = (None, "not measured")
     if "batch" in legs and not args.no_traffic:

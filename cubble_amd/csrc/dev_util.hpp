@@ -174,7 +174,7 @@ __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfi
 // lane's loads of one chunk are independent, so their latencies overlap
 // instead of adding up entry by entry.  Past the end of the list a lane
 // re-reads its own record and skips the body.
-constexpr int kGather = 8;
+constexpr int kGather = 4;
 
 // The buffers live in device global memory; pointers read from the foam
 // header are generic, so loads through them compile to flat loads, whose
