@@ -103,6 +103,10 @@ def leg_batch(ca, args, ip):
     b = ca.Batch(cfgs)
     log("batch: created 256 foams (%d bubbles) in %.1f s" %
         (sum(c.num_start for c in cfgs), time.perf_counter() - t))
+    # warm up first: the shapes are then tuned on steps that include searches
+    # and removals, as in the timed run
+    if args.warmup > 0:
+        b.run_steps(args.warmup, ip)
     t = time.perf_counter()
     if args.tuning:
         # fixed shapes (no timing trials): bubbles_per_thread,search_cells,graph_steps
@@ -113,8 +117,6 @@ def leg_batch(ca, args, ip):
         tune = b.autotune(ip, trial_steps=args.tune_steps, repeats=2,
                           cache_path=args.tuning_cache)
     log("batch: launch shapes in %.1f s: %s" % (time.perf_counter() - t, json.dumps(tune)))
-    if args.warmup > 0:
-        b.run_steps(args.warmup, ip)
     s0 = [b.scalars(k) for k in range(len(cfgs))]
     t0 = time.perf_counter()
     res = b.run_steps(args.steps, ip)
@@ -249,7 +251,9 @@ def read_counters(folder, counter):
             for row in csv.DictReader(f):
                 if row.get("Counter_Name") != counter:
                     continue
-                out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+                # "ns::kernel(args)" -> "kernel"
+                name = row["Kernel_Name"].split("(")[0].split("::")[-1]
+                out.setdefault(name, []).append(float(row["Counter_Value"]))
     return out
 
 
@@ -280,8 +284,7 @@ def measure_traffic(kernel):
         vals = read_counters(folder, counter)
         shutil.rmtree(folder, ignore_errors=True)
         copy = [v for k, v in vals.items() if "copy" in k.lower()]
-        mine = vals.get(kernel) or next((v for k, v in vals.items() if k.startswith(kernel + "(")),
-                                        None)
+        mine = vals.get(kernel)
         if r.returncode not in (0, -11, 139) and not mine:
             return None, "rocprofv3 %s pass failed (rc %d)" % (counter, r.returncode)
         if not copy or not mine:
@@ -329,10 +332,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed steps of the batched workload")
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--legs", default="batch,plain,foam,serve")
     ap.add_argument("--foam-steps", type=int, default=2000)
-    ap.add_argument("--tune-steps", type=int, default=32)
+    ap.add_argument("--tune-steps", type=int, default=64)
     ap.add_argument("--tuning-cache", default=None)
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes bubbles_per_thread,search_cells,graph_steps "

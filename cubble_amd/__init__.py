@@ -452,7 +452,7 @@ class Simulator:
                                           int(n), ctypes.byref(r)))
         return _res(r)
 
-    def autotune(self, ip=None, trial_steps=32, repeats=2, cache_path=None, ignore_cache=False):
+    def autotune(self, ip=None, trial_steps=64, repeats=2, cache_path=None, ignore_cache=False):
         """Pick launch shapes by timing them on this foam; the state is
         restored afterwards.  Returns the chosen shapes and the rates."""
         return _autotune(lib().cubble_autotune, self._h, ip, trial_steps, repeats, cache_path,
@@ -614,7 +614,7 @@ class Batch:
         sc = self.scalars(k)
         return _read_state(sc, lambda v: self._call(lib().cubble_batch_get_state(self._h, int(k), v))), sc
 
-    def autotune(self, ip=None, trial_steps=32, repeats=2, cache_path=None, ignore_cache=False):
+    def autotune(self, ip=None, trial_steps=64, repeats=2, cache_path=None, ignore_cache=False):
         """Pick launch shapes by timing them on these foams; the state is
         restored afterwards.  Returns the chosen shapes and the rates."""
         return _autotune(lib().cubble_batch_autotune, self._h, ip, trial_steps, repeats,

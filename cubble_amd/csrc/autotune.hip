@@ -3,7 +3,9 @@
 // Three shapes matter on this path: bubbles per thread of the bubble-space
 // kernels (block count against per-thread work and the number of block
 // partials to reduce), cells per neighbour-search block (the search tile:
-// how many cells, one wave each, share a workgroup), and attempt groups per
+// how many cells, one wave each, share a workgroup: small tiles give the few
+// searching foams of a batch enough workgroups, large ones cheapen the idle
+// launch over every foam's tiles), and attempt groups per
 // captured graph replay (host synchronisations against the groups a replay
 // wastes after every foam reached its step limit).
 //
@@ -77,7 +79,7 @@ int time_shapes(Engine &e, const Snapshot &snap, uint32_t flags, const Tuning &t
 
 int autotune(Engine &e, uint32_t flags, const cubble_autotune_opts *o, cubble_autotune_report *rep) {
     if (e.foams.empty()) return e.fail(CUBBLE_E_INVALID, "nothing to tune");
-    const int steps = o && o->trial_steps > 0 ? o->trial_steps : 32;
+    const int steps = o && o->trial_steps > 0 ? o->trial_steps : 64;
     const int repeats = o && o->repeats > 0 ? o->repeats : 2;
     const char *cache = o ? o->cache_path : nullptr;
     const std::string key = cache_key(e);

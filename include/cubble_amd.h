@@ -221,7 +221,7 @@ int cubble_create_tuned(const cubble_config *cfg, const cubble_tuning *tuning, c
  * simulation).  With cache_path, shapes are looked up in / stored to a JSON
  * file keyed by device and batch make-up.  No counterpart in the reference. */
 typedef struct cubble_autotune_opts {
-    int32_t trial_steps;    /* accepted steps timed per candidate (0 = 32)   */
+    int32_t trial_steps;    /* accepted steps timed per candidate (0 = 64)   */
     int32_t repeats;        /* timings per candidate, fastest kept (0 = 2)   */
     int32_t ignore_cache;   /* time the candidates even on a cache hit       */
     const char *cache_path; /* JSON cache file or NULL                       */
