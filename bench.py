@@ -18,6 +18,9 @@ HIP graphs with autotuned launch shapes, K steps each.  The same line carries:
   serving                Poisson requests (50/s, 3D, 1e3..1e6 bubbles, 500
                          steps or half dissolved) through the continuous-
                          batching request loop: p50 / p99 latency, throughput;
+  mixed_precision_error  the 3D 1e7-bubble foam: per-quantity error of a
+                         mixed-precision attempt against FP64 from the same
+                         state, and the drift of a 500-step mixed run;
   cpu_baseline           the OpenMP FP64 restatement on the host cores, on a
                          bounded sample of the headline workload.
 Prints one JSON line.
@@ -186,6 +189,66 @@ def leg_foam(ca, args, ip):
             "counters": counters([r0], [r1]), "roofline": rf}
 
 
+def scaled_err(gpu, ref, scale):
+    """max_i |gpu_i - ref_i| / max(|ref_i|, scale): relative error, judged
+    against the quantity's scale where its value cancels to about zero."""
+    import numpy as np
+
+    gpu = np.asarray(gpu, float)
+    ref = np.asarray(ref, float)
+    if gpu.size == 0:
+        return 0.0
+    return float(np.max(np.abs(gpu - ref) / np.maximum(np.abs(ref), scale)))
+
+
+def leg_mixed_error(ca, args, ip):
+    """Mixed precision against FP64 on the 3D 1e7-bubble foam (phi 0.9).
+    Per step: the FP64 state after `mixed_steps` steps is loaded into a
+    mixed-precision simulator and one attempt of each is compared, quantity
+    by quantity.  Over the run: the mixed and FP64 runs of the same seed,
+    `mixed_steps` steps each, compared on count, mean radius and gas
+    volume."""
+    import numpy as np
+
+    c64 = ca.Config.workload("foam3d_1e7")
+    cmx = dataclasses.replace(c64, precision=ca.PREC_MIXED)
+    a = ca.Simulator(c64)
+    b = ca.Simulator(cmx)
+    a.run_steps(args.mixed_steps, ip)
+    b.run_steps(args.mixed_steps, ip)
+    sa, sca = a.get_state()
+    sb, scb = b.get_state()
+
+    def vol(r):
+        return float(np.sum(4.0 / 3.0 * np.pi * np.asarray(r) ** 3))
+
+    traj = {"count_fp64": int(sca["count"]), "count_mixed": int(scb["count"]),
+            "mean_radius_rel_diff": abs(float(np.mean(sb["x"][3])) / float(np.mean(sa["x"][3])) - 1),
+            "gas_volume_rel_diff": abs(vol(sb["x"][3]) / vol(sa["x"][3]) - 1)}
+    b.set_state(sa, sca)
+    a.step(ip)
+    b.step(ip)
+    ga, gb = a.get_attempt(), b.get_attempt()
+    f0 = sca["f0"]
+    s_r = max(float(np.max(np.abs(ga["vp"][3]))), 1e-300)
+    s_ov = max(float(np.max(np.abs(ga["ov"]))), 1e-300)
+    per_step = {
+        "velocity": max(scaled_err(gb["vp"][k], ga["vp"][k], f0) for k in range(3)),
+        "drdt": scaled_err(gb["vp"][3], ga["vp"][3], s_r),
+        "overlap_area": scaled_err(gb["ov"], ga["ov"], s_ov),
+        "position": max(scaled_err(gb["xc"][k], ga["xc"][k], 1.0) for k in range(3)),
+        "radius": scaled_err(gb["xc"][3], ga["xc"][3], 1.0),
+    }
+    a.close()
+    b.close()
+    log("mixed error: %s %s" % (json.dumps(per_step), json.dumps(traj)))
+    return {"workload": "3D periodic foam, 10,000,000 bubbles, log-normal radii sigma 0.15, "
+                        "phi 0.9", "steps": args.mixed_steps,
+            "per_step_vs_fp64": per_step, "metric": "max |mixed - fp64| / max(|fp64|, scale); "
+            "scale = f0 for velocities, max|q| for dR/dt and overlap, 1 for positions and radii",
+            "run_vs_fp64": traj}
+
+
 def leg_serve(ca, args, ip):
     reqs = ca.serving_requests(args.serve_requests, rate_per_s=50.0, precision=ca.PREC_MIXED)
     srv = ca.Server(hbm_budget_bytes=args.serve_budget_gib << 30, round_steps=args.serve_round,
@@ -333,7 +396,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed steps of the batched workload")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--legs", default="batch,plain,foam,serve")
+    ap.add_argument("--legs", default="batch,plain,foam,serve,mixed-error")
+    ap.add_argument("--mixed-steps", type=int, default=500)
     ap.add_argument("--foam-steps", type=int, default=2000)
     ap.add_argument("--tune-steps", type=int, default=64)
     ap.add_argument("--tuning-cache", default=None)
@@ -404,6 +468,8 @@ def main():
         out["foam3d_1e6_fp64"] = leg_foam(ca, args, ip)
     if "serve" in legs:
         out["serving"] = leg_serve(ca, args, ip)
+    if "mixed-error" in legs:
+        out["mixed_precision_error"] = leg_mixed_error(ca, args, ip)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfgs, args.steps, args.cpu_budget)
     print(json.dumps(out), flush=True)
