@@ -112,7 +112,7 @@ def leg_batch(ca, args, ip):
         b.run_steps(args.warmup, ip)
     t = time.perf_counter()
     if args.tuning:
-        # fixed shapes (no timing trials): bubbles_per_thread,search_cells,graph_steps
+        # fixed shapes (no timing trials): bubbles_per_thread,search_tile,graph_steps
         shapes = dict(zip(ca.TUNING_KEYS, [int(v) for v in args.tuning.split(",")]))
         b.retune(shapes)
         tune = {"chosen": shapes, "from_cache": False, "candidates_timed": 0, "fixed": True}
@@ -402,7 +402,7 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=64)
     ap.add_argument("--tuning-cache", default=None)
     ap.add_argument("--tuning", default=None,
-                    help="fixed launch shapes bubbles_per_thread,search_cells,graph_steps "
+                    help="fixed launch shapes bubbles_per_thread,search_tile,graph_steps "
                          "instead of autotuning")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-budget-gib", type=int, default=64)

@@ -108,11 +108,11 @@ class _KernelTimes(ctypes.Structure):
 
 
 class _Tuning(ctypes.Structure):
-    _fields_ = [("bubbles_per_thread", ctypes.c_int32), ("search_cells", ctypes.c_int32),
+    _fields_ = [("bubbles_per_thread", ctypes.c_int32), ("search_tile", ctypes.c_int32),
                 ("graph_steps", ctypes.c_int32)]
 
 
-TUNING_KEYS = ("bubbles_per_thread", "search_cells", "graph_steps")
+TUNING_KEYS = ("bubbles_per_thread", "search_tile", "graph_steps")
 
 
 def _tuning(t):

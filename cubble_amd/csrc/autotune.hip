@@ -2,10 +2,10 @@
 //
 // Three shapes matter on this path: bubbles per thread of the bubble-space
 // kernels (block count against per-thread work and the number of block
-// partials to reduce), cells per neighbour-search block (the search tile:
-// how many cells, one wave each, share a workgroup: small tiles give the few
-// searching foams of a batch enough workgroups, large ones cheapen the idle
-// launch over every foam's tiles), and attempt groups per
+// partials to reduce), bubbles per neighbour-search workgroup (the search
+// tile: small tiles give the few searching foams of a batch enough
+// workgroups, large ones cheapen the idle launch over every foam's tiles),
+// and attempt groups per
 // captured graph replay (host synchronisations against the groups a replay
 // wastes after every foam reached its step limit).
 //
@@ -88,12 +88,12 @@ int autotune(Engine &e, uint32_t flags, const cubble_autotune_opts *o, cubble_au
     if (cache && (!o || !o->ignore_cache) && tuning_cache_lookup(cache, key, shape)) {
         Tuning t;
         t.ipt = shape[0];
-        t.scells = shape[1];
+        t.stile = shape[1];
         t.groups = shape[2];
         const int rc = e.retune(t);
         if (rc) return rc;
         r.chosen.bubbles_per_thread = t.ipt;
-        r.chosen.search_cells = t.scells;
+        r.chosen.search_tile = t.stile;
         r.chosen.graph_steps = t.groups;
         r.from_cache = 1;
         if (rep) *rep = r;
@@ -124,15 +124,15 @@ int autotune(Engine &e, uint32_t flags, const cubble_autotune_opts *o, cubble_au
     }
     r.start_rate = best_rate;
     const std::vector<int> ipts = {1, 2, 4, 8};
-    const std::vector<int> tiles = {4, 8, 16, 32, 64};
+    const std::vector<int> tiles = {32, 64, 128, 256, 512};
     const std::vector<int> groups = {4, 8, 16, 32};
     for (int axis = 0; axis < 3 && !rc; ++axis) {
         const std::vector<int> &cand = 0 == axis ? ipts : 1 == axis ? tiles : groups;
         const Tuning base = best;
         for (int v : cand) {
             Tuning t = base;
-            int &field = 0 == axis ? t.ipt : 1 == axis ? t.scells : t.groups;
-            const int cur = 0 == axis ? base.ipt : 1 == axis ? base.scells : base.groups;
+            int &field = 0 == axis ? t.ipt : 1 == axis ? t.stile : t.groups;
+            const int cur = 0 == axis ? base.ipt : 1 == axis ? base.stile : base.groups;
             if (v == cur) continue;
             double rate = 0.0;
             field = v;
@@ -148,12 +148,12 @@ int autotune(Engine &e, uint32_t flags, const cubble_autotune_opts *o, cubble_au
     e.free_snapshot(snap);
     if (rc) return rc;
     r.chosen.bubbles_per_thread = best.ipt;
-    r.chosen.search_cells = best.scells;
+    r.chosen.search_tile = best.stile;
     r.chosen.graph_steps = best.groups;
     r.best_rate = best_rate;
     if (cache) {
         shape[0] = best.ipt;
-        shape[1] = best.scells;
+        shape[1] = best.stile;
         shape[2] = best.groups;
         std::string err;
         if ((rc = tuning_cache_store(cache, key, shape, best_rate, err))) return e.fail(rc, err);

@@ -79,9 +79,10 @@ size_t carve_foam(Foam &F, char *base) {
     S.cell_cursor = L.take<int32_t>((size_t)F.cell_cap);
     S.cell_start = L.take<int32_t>((size_t)F.cell_cap + 1);
     // block partials sized for the smallest launch shapes (one bubble per
-    // thread, kMinSearchCells per search block), so retuning never reallocates
+    // thread, kMinSearchTile bubbles per search tile), so retuning never
+    // reallocates
     const size_t nblk_max = ((size_t)F.cap + kBlock - 1) / kBlock;
-    const size_t sblk_max = ((size_t)F.cell_cap + kMinSearchCells - 1) / kMinSearchCells;
+    const size_t sblk_max = (size_t)F.cap / kMinSearchTile + 1;
     S.part = L.take<double>(nblk_max * kPartD);
     S.part_i = L.take<int32_t>(nblk_max * kPartI);
     S.cpart = L.take<int32_t>((size_t)F.cblk);
@@ -94,8 +95,10 @@ void shape_foam(Foam &F, const Tuning &t) {
     const int cap = F.cap;
     F.ipt = t.ipt > 0 ? t.ipt : std::max(1, (cap + kBlock * kMaxBlocks - 1) / (kBlock * kMaxBlocks));
     F.nblk = std::max(1, (cap + kBlock * F.ipt - 1) / (kBlock * F.ipt));
-    F.scells = std::max(kMinSearchCells, t.scells > 0 ? t.scells : 32);
-    F.sblk = std::max(1, (F.cell_cap + F.scells - 1) / F.scells);
+    F.stile = std::max(kMinSearchTile, t.stile > 0 ? t.stile : 256);
+    // tile t holds the cells whose first bubble is in [t B, t B + B); bubbles
+    // are below the capacity, so t runs to cap / B
+    F.sblk = F.cap / F.stile + 1;
 }
 
 void size_foam(Foam &F, int cap, int nbr_cap, int cell_cap, const Tuning &t) {
@@ -423,7 +426,7 @@ int Engine::restore(const Snapshot &s) {
         f.nbr_cap = r.h.nbr_cap;
         f.ipt = r.h.ipt;
         f.nblk = r.h.nblk;
-        f.scells = r.h.scells;
+        f.stile = r.h.stile;
         f.sblk = r.h.sblk;
         r.h = f;
     }

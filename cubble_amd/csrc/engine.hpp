@@ -46,12 +46,12 @@ class DeviceArena {
 
 struct Tuning {
     int ipt = 0;       // bubbles per thread (0 = from the size)
-    int scells = 32;   // cells per neighbour-search block
+    int stile = 256;   // bubbles per neighbour-search tile
     int groups = 16;   // attempt groups per captured graph replay
 };
 
-// Smallest search tile: the search block has four waves, one cell each.
-constexpr int kMinSearchCells = 4;
+// Smallest neighbour-search tile, in bubbles.
+constexpr int kMinSearchTile = 32;
 
 // Device copies of every foam's buffers and host copies of their headers,
 // taken by Engine::snapshot and put back by Engine::restore.

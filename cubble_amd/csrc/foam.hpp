@@ -69,7 +69,7 @@ struct Scratch {
     double *part;        // bubble-space block partials [nblk][kPartD]
     int32_t *part_i;     // bubble-space block partials [nblk][kPartI]
     int32_t *cpart;      // cell-space block partials (cell-count sums)
-    int32_t *cpart_i;    // neighbour-search block partials [sblk][kPartI]
+    int32_t *cpart_i;    // neighbour-search tile partials [sblk][kPartI]
     int32_t *del_list;   // sorted deleted indices
     int32_t *plan_src;   // source (tail survivor) of hole k
     int32_t *moved_to;   // new slot of tail survivor (old index - new count)
@@ -112,8 +112,8 @@ struct Foam {
     int32_t ipt;         // bubbles per thread in bubble-space kernels
     int32_t cblk;        // cell-space blocks
     int32_t cipt;        // cells per thread in cell-space kernels
-    int32_t sblk;        // neighbour-search blocks
-    int32_t scells;      // cells per neighbour-search block
+    int32_t sblk;        // neighbour-search tiles (workgroups)
+    int32_t stile;       // bubbles per neighbour-search tile
     int32_t precision;
 };
 

@@ -457,7 +457,7 @@ bool tuning_cache_lookup(const char *path, const std::string &key, int shape[3])
     const JVal *e = root.get(key.c_str());
     if (!e) return false;
     double v[3];
-    if (!jnum(*e, {"bubbles_per_thread"}, v[0]) || !jnum(*e, {"search_cells"}, v[1]) ||
+    if (!jnum(*e, {"bubbles_per_thread"}, v[0]) || !jnum(*e, {"search_tile"}, v[1]) ||
         !jnum(*e, {"graph_steps"}, v[2]))
         return false;
     for (int k = 0; k < 3; ++k) shape[k] = (int)v[k];
@@ -473,12 +473,12 @@ int tuning_cache_store(const char *path, const std::string &key, const int shape
         for (const auto &kv : root.obj) {
             double v[4] = {0, 0, 0, 0};
             if (!jnum(kv.second, {"bubbles_per_thread"}, v[0]) ||
-                !jnum(kv.second, {"search_cells"}, v[1]) || !jnum(kv.second, {"graph_steps"}, v[2]))
+                !jnum(kv.second, {"search_tile"}, v[1]) || !jnum(kv.second, {"graph_steps"}, v[2]))
                 continue;
             jnum(kv.second, {"rate"}, v[3]);
             char buf[256];
             std::snprintf(buf, sizeof(buf),
-                          "{\"bubbles_per_thread\": %d, \"search_cells\": %d, \"graph_steps\": %d, "
+                          "{\"bubbles_per_thread\": %d, \"search_tile\": %d, \"graph_steps\": %d, "
                           "\"rate\": %.6g}",
                           (int)v[0], (int)v[1], (int)v[2], v[3]);
             entries[kv.first] = buf;
@@ -486,7 +486,7 @@ int tuning_cache_store(const char *path, const std::string &key, const int shape
     }
     char buf[256];
     std::snprintf(buf, sizeof(buf),
-                  "{\"bubbles_per_thread\": %d, \"search_cells\": %d, \"graph_steps\": %d, "
+                  "{\"bubbles_per_thread\": %d, \"search_tile\": %d, \"graph_steps\": %d, "
                   "\"rate\": %.6g}",
                   shape[0], shape[1], shape[2], rate);
     entries[key] = buf;

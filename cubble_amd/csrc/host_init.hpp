@@ -30,7 +30,7 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err);
 void config_default(cubble_config &cfg);
 
 // Launch shapes chosen by the autotuner, kept in a JSON file keyed by the
-// device and the batch's make-up: {key: {bubbles_per_thread, search_cells,
+// device and the batch's make-up: {key: {bubbles_per_thread, search_tile,
 // graph_steps, rate}}.  Lookup returns false when the file or key is absent.
 bool tuning_cache_lookup(const char *path, const std::string &key, int shape[3]);
 int tuning_cache_store(const char *path, const std::string &key, const int shape[3], double rate,

@@ -15,7 +15,7 @@ struct Grid {
     int nb_bub;
     const int2 *bt_cell;
     int nb_cell;
-    const int2 *bt_scell;  // neighbour-search grid (Foam::scells cells per block)
+    const int2 *bt_scell;  // neighbour-search grid (Foam::stile bubbles per tile)
     int nb_scell;
     int nfoams;
     hipStream_t stream;

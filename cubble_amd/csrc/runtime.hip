@@ -59,15 +59,15 @@ Tuning tuning_of(const cubble_tuning *t) {
     Tuning u;
     if (t) {
         u.ipt = t->bubbles_per_thread;
-        if (t->search_cells > 0) u.scells = t->search_cells;
+        if (t->search_tile > 0) u.stile = t->search_tile;
         if (t->graph_steps > 0) u.groups = t->graph_steps;
     }
     return u;
 }
 
 int retune(Engine &e, const cubble_tuning *t) {
-    if (!t || t->bubbles_per_thread < 0 || t->search_cells < 0 || t->graph_steps < 0 ||
-        (t->search_cells > 0 && t->search_cells < kMinSearchCells))
+    if (!t || t->bubbles_per_thread < 0 || t->search_tile < 0 || t->graph_steps < 0 ||
+        (t->search_tile > 0 && t->search_tile < kMinSearchTile))
         return e.fail(CUBBLE_E_INVALID, "bad launch shapes");
     return e.retune(tuning_of(t));
 }

@@ -465,8 +465,31 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     }
 }
 
-// Search grid: block b holds cells [s b, s b + s), s = scells; wave w takes
-// cells s b + w, s b + w + 4, ...
+// First index c in [0, m] with a[c] >= key, or m + 1 when there is none; a is
+// non-decreasing.  The workgroup probes kBlock evenly spaced positions of
+// the remaining range per round, so the range shrinks kBlock-fold each round
+// (two rounds for ~65,000 cells).  Every thread returns the same value.
+__device__ __forceinline__ int block_lower_bound(const int *__restrict__ a, int m, int key) {
+    int lo = 0, hi = m + 1;  // the answer lies in [lo, hi]; position m + 1 counts as >= key
+    while (lo < hi) {
+        const int step = (hi - lo + kBlock) / kBlock;  // ceil((hi - lo + 1) / kBlock)
+        const int p = lo + (int)threadIdx.x * step;
+        const bool probe = p <= hi;
+        const bool ge = probe && (p > m || a[p] >= key);
+        const int first_ge = -block_max_i(ge ? -p : -(hi + 1));   // min over true probes
+        const int last_lt = block_max_i(probe && !ge ? p : lo - 1);
+        hi = first_ge <= hi ? first_ge : hi;
+        lo = last_lt + 1;
+    }
+    return lo;
+}
+
+// Search grid: tile t of a foam holds the cells whose first bubble lies in
+// [t B, t B + B), B = stile bubbles, found by a lower bound on the cell
+// starts; each cell belongs to exactly one tile.  Wave w takes the tile's
+// cells w, w + 4, ...  Tiles sized in bubbles keep the grid proportional to
+// the bubbles of a foam whatever its cell occupancy (2D cells hold a few
+// bubbles, 3D ones a few tens).
 __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt) {
     const Ctx cx = cell_ctx(foams, bt);
     Foam *F = cx.F;
@@ -475,8 +498,12 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
     const Scratch S = F->s;
     const int ncell = F->d.num_cells;
     const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
-    const int base = cx.lb * F->scells;
-    const int end = min(base + F->scells, ncell);
+    const int b0 = cx.lb * F->stile;
+    int base = 0, end = 0;  // cells of this tile (none past the bubble count)
+    if (b0 < F->d.count) {
+        base = block_lower_bound(S.cell_start, ncell, b0);
+        end = min(block_lower_bound(S.cell_start, ncell, b0 + F->stile), ncell);
+    }
     const size_t cap = (size_t)F->cap;
     int sum_nn = 0, max_nn = 0, ovf = 0;
     // the fast path needs 5 or more cells along every periodic axis

@@ -208,7 +208,8 @@ int cubble_set_state(cubble_sim *sim, const cubble_state_view *view, const cubbl
  * src/data_definitions.h:28-29). */
 typedef struct cubble_tuning {
     int32_t bubbles_per_thread;  /* bubble-space kernels (0 = from the foam size)   */
-    int32_t search_cells;        /* cells per neighbour-search block (0 = 32, >= 4) */
+    int32_t search_tile;         /* bubbles per neighbour-search workgroup
+                                    (0 = 256, >= 32)                        */
     int32_t graph_steps;         /* attempt groups per graph replay (0 = 16)        */
 } cubble_tuning;
 

@@ -74,7 +74,7 @@ def test_batch_many_small_foams_and_tuning():
         cfgs.append(config(dim, n, phi=float(rng.uniform(0.8, 0.95)),
                            kappa=float(10 ** rng.uniform(-3, -1)), seed=500 + k))
     a = ca.Batch(cfgs)
-    b = ca.Batch(cfgs, tuning={"bubbles_per_thread": 2, "search_cells": 16})
+    b = ca.Batch(cfgs, tuning={"bubbles_per_thread": 2, "search_tile": 128})
     ra = a.run_steps(25, GAS)
     rb = b.run_steps(25, GAS)
     for k in range(len(cfgs)):
@@ -126,18 +126,18 @@ def test_autotune_restores_state_and_chosen_shapes_match_alone(tmp_path):
 
 def test_retune_live_simulation_equals_created_with_shapes():
     cfg = config(3, 30000, seed=21)
-    shapes = {"bubbles_per_thread": 4, "search_cells": 8, "graph_steps": 4}
+    shapes = {"bubbles_per_thread": 4, "search_tile": 64, "graph_steps": 4}
     a = ca.Simulator(cfg)
     a.run_steps(10, GAS)
     a.retune(shapes)
     b = ca.Simulator(cfg, tuning=shapes)
-    b.retune({"bubbles_per_thread": 0, "search_cells": 32, "graph_steps": 16})
+    b.retune({"bubbles_per_thread": 0, "search_tile": 256, "graph_steps": 16})
     b.run_steps(10, GAS)
     b.retune(shapes)
     a.run_steps(20, GAS)
     b.run_steps(20, GAS)
     assert_same(a.get_state(), b.get_state())
     with pytest.raises(ca.CubbleError):
-        a.retune({"search_cells": 2})
+        a.retune({"search_tile": 8})
     a.close()
     b.close()
