@@ -23,6 +23,8 @@
 //   k_s_fin     : one block per foam makes the reordered buffers current
 //                 (the pointer rotation of simulator.cu:96-147) and totals the
 //                 pair count
+#include <type_traits>
+
 #include "dev_util.hpp"
 #include "launch.hpp"
 
@@ -265,17 +267,26 @@ __device__ __forceinline__ XRuns x_runs(const AxisCells &X) {
     return r;
 }
 
-// Wave-uniform append of the ballot `m` of candidates j to own bubble i's
-// list (lane `ii` of the group keeps the count).
-__device__ __forceinline__ void append_hits(unsigned long long m, bool hit, int ii, int i, int j,
-                                            unsigned long long lt, int *__restrict__ nbr,
-                                            size_t cap, int nbr_cap, int &cnt) {
-    const int base = __shfl(cnt, ii);
-    if (hit) {
-        const int pos = base + __popcll(m & lt);
-        if (pos < nbr_cap) nbr[(size_t)pos * cap + i] = j;
+// Wave-uniform append of the ballot `m` (non-zero) of candidates j to own
+// bubble i's list.  Lane `ii` of the group (ii is wave-uniform) keeps the
+// count: it is read with readlane and updated on the scalar unit.  A hit
+// lane's slot is the count plus the hits in the lanes below it (mbcnt); the
+// hit itself is re-derived from `m`, so the comparison result need not stay
+// live across the branch.
+__device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i, int j,
+                                            int *__restrict__ nbr_, unsigned cap, int nbr_cap,
+                                            int &cnt) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int base = __builtin_amdgcn_readlane(cnt, ii);
+    if ((m >> lane) & 1ull) {
+        const int below = (int)__builtin_amdgcn_mbcnt_hi(
+            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        const int pos = base + below;
+        auto nbr = (__attribute__((address_space(1))) int *)(nbr_ + i);
+        if (pos < nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;
     }
-    if ((int)(threadIdx.x & 63) == ii) cnt += __popcll(m);
+    const int upd = base + __popcll(m);  // scalar
+    cnt = lane == ii ? upd : cnt;
 }
 
 __device__ __forceinline__ void finish_group(int cb, int nb, int cnt, int nbr_cap,
@@ -322,9 +333,8 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
     const AxisCells Z = 3 == D ? axis_cells(ccz, cd[2], c.wall[2] != 0) : AxisCells{0, 0, 0, 1};
     const XRuns xr = x_runs(X);
     const double skin = c.skin;
-    const unsigned long long lt = (1ull << lane) - 1ull;
     for (int cb = i0; cb < i1; cb += 64) {
-        const int nb = min(64, i1 - cb);
+        const int nb = uniform_i(min(64, i1 - cb));
         int cnt = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -339,6 +349,7 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
                     for (int jb = j0; jb < j1; jb += 64) {
                         const int j = jb + lane;
                         const bool valid = j < j1;
+                        const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
                         const int jl = valid ? j : j0;
                         const D4 pj = D4{x4[jl].x, x4[jl].y, 3 == D ? x4[jl].z : 0.0, x4[jl].r};
                         for (int ii = 0; ii < nb; ++ii) {
@@ -348,9 +359,12 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
                             const double maxd = pr + pj.r + skin;
                             double d[3];
                             wrapped_diff<D>(c, px, py, pz, pj.x, pj.y, pj.z, d);
-                            const bool hit = valid && j != i && sq_len(d) < maxd * maxd;
-                            const unsigned long long m = __ballot(hit);
-                            if (m) append_hits(m, hit, ii, i, j, lt, nbr, cap, nbr_cap, cnt);
+                            // one ballot per comparison: each is the comparison's own
+                            // lane mask, with no boolean materialised in a register
+                            const unsigned long long m =
+                                vmask & __builtin_amdgcn_ballot_w64(j != i) &
+                                __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
+                            if (m) append_hits(m, ii, i, j, nbr, (unsigned)cap, nbr_cap, cnt);
                         }
                     }
                 }
@@ -370,9 +384,12 @@ constexpr int kMaxRuns = 18;  // 3 z-rows x 3 y-rows x 2 x-runs
 // With 5 or more cells per axis the minimum image is decided by the cells
 // alone: a candidate in a neighbouring cell that does not cross the boundary
 // is within 2/5 of the box and needs no wrap (|d| < L - |d|), one across the
-// boundary is at least 3/5 away and always takes the image L - |d| -- the
-// same operations as the general minimum-image difference, applied without
-// its test.  Same order, same lists, same rounding as search_cell_general.
+// boundary is at least 3/5 away and always takes the image, whose sign the
+// cells fix: d - L for a candidate across the low end (d > 0), d + L across
+// the high end.  Rounding is symmetric, so fl(d - L) = -fl(L - |d|) and
+// fl(d + L) = fl(L - |d|): the values of the general minimum-image
+// difference, without its test.  Same order, same lists, same rounding as
+// search_cell_general.
 // `cell` must be wave-uniform.
 template <int D>
 __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4_,
@@ -394,8 +411,11 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     const AxisCells Y = axis_cells(ccy, cd[1], c.wall[1] != 0);
     const AxisCells Z = 3 == D ? axis_cells(ccz, cd[2], c.wall[2] != 0) : AxisCells{0, 0, 0, 1};
     const XRuns xr = x_runs(X);
-    auto far = [](int a, int b) { return a - b > 1 || b - a > 1; };
-    const int fx0 = far(xr.a0, ccx) ? 1 : 0, fx1 = far(xr.a1, ccx) ? 1 : 0;
+    // 0: neighbour cell a of own cell b does not cross the boundary; 1: it
+    // lies across the low end (d > 0, image d - L); 2: across the high end
+    // (d < 0, image d + L)
+    auto side = [](int a, int b) { return b - a > 1 ? 1 : (a - b > 1 ? 2 : 0); };
+    const int fx0 = side(xr.a0, ccx), fx1 = side(xr.a1, ccx);
     int j0[kMaxRuns], len[kMaxRuns], pre[kMaxRuns], fl[kMaxRuns];
     int total = 0;
 #pragma unroll
@@ -405,7 +425,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             const bool row_ok = a < Z.n && b < Y.n;
             const int zc = Z.at(a), yc = Y.at(b);
             const int row = zc * cd[0] * cd[1] + yc * cd[0];
-            const int fyz = (far(yc, ccy) ? 2 : 0) | (3 == D && far(zc, ccz) ? 4 : 0);
+            const int fyz = (side(yc, ccy) << 2) | (3 == D ? side(zc, ccz) << 4 : 0);
 #pragma unroll
             for (int part = 0; part < 2; ++part) {
                 const int s = (a * 3 + b) * 2 + part;
@@ -422,9 +442,8 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     }
     const double skin = c.skin;
     const double L[3] = {c.interval[0], c.interval[1], c.interval[2]};
-    const unsigned long long lt = (1ull << lane) - 1ull;
     for (int cb = i0; cb < i1; cb += 64) {
-        const int nb = min(64, i1 - cb);
+        const int nb = uniform_i(min(64, i1 - cb));
         int cnt = 0;
         for (int p0 = 0; p0 < total; p0 += 64) {
             const int p = p0 + lane;
@@ -438,28 +457,41 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                 }
             }
             const bool valid = j >= 0;
+            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
             const int jl = valid ? j : i0;
             const D4 pj = D4{x4[jl].x, x4[jl].y, 3 == D ? x4[jl].z : 0.0, x4[jl].r};
-            const bool anyflip = __ballot(valid && f != 0) != 0ull;
-            for (int ii = 0; ii < nb; ++ii) {
-                const int i = cb + ii;
-                const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
-                const double pz = 3 == D ? x4s[i].z : 0.0;
-                double d[3] = {px - pj.x, py - pj.y, 3 == D ? pz - pj.z : 0.0};
-                if (anyflip) {
+            const bool anyflip = __builtin_amdgcn_ballot_w64(valid && f != 0) != 0ull;
+            // image shift of this lane's candidate per axis: 0, +L or -L
+            double sh[3];
 #pragma unroll
-                    for (int k = 0; k < D; ++k) {
-                        const double a = d[k] < 0 ? -d[k] : d[k];
-                        const double t = L[k] - a;
-                        const double w = d[k] < 0 ? t : -t;
-                        d[k] = ((f >> k) & 1) ? w : d[k];
-                    }
-                }
-                const double maxd = pr + pj.r + skin;
-                const bool hit = valid && j != i && sq_len(d) < maxd * maxd;
-                const unsigned long long m = __ballot(hit);
-                if (m) append_hits(m, hit, ii, i, j, lt, nbr, cap, nbr_cap, cnt);
+            for (int k = 0; k < 3; ++k) {
+                const int fk = (f >> (2 * k)) & 3;
+                sh[k] = 1 == fk ? L[k] : (2 == fk ? -L[k] : 0.0);
             }
+            // the own-bubble loop in two versions, so that chunks with no
+            // boundary crossing carry no image arithmetic
+            auto own_loop = [&](auto flip) {
+                for (int ii = 0; ii < nb; ++ii) {
+                    const int i = cb + ii;
+                    const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
+                    const double pz = 3 == D ? x4s[i].z : 0.0;
+                    double d[3] = {px - pj.x, py - pj.y, 3 == D ? pz - pj.z : 0.0};
+                    if (decltype(flip)::value) {
+                        // d - (+-L) rounds exactly as -+(L - |d|); d - 0 is d
+#pragma unroll
+                        for (int k = 0; k < D; ++k) d[k] = d[k] - sh[k];
+                    }
+                    const double maxd = pr + pj.r + skin;
+                    // one ballot per comparison: each is the comparison's own
+                    // lane mask, with no boolean materialised in a register
+                    const unsigned long long m =
+                        vmask & __builtin_amdgcn_ballot_w64(j != i) &
+                        __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
+                    if (m) append_hits(m, ii, i, j, nbr, (unsigned)cap, nbr_cap, cnt);
+                }
+            };
+            if (anyflip) own_loop(std::true_type{});
+            else own_loop(std::false_type{});
         }
         finish_group(cb, nb, cnt, nbr_cap, nn, sum_nn, max_nn, ovf);
     }
