@@ -416,10 +416,12 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     // (d < 0, image d + L)
     auto side = [](int a, int b) { return b - a > 1 ? 1 : (a - b > 1 ? 2 : 0); };
     const int fx0 = side(xr.a0, ccx), fx1 = side(xr.a1, ccx);
-    int j0[kMaxRuns], len[kMaxRuns], pre[kMaxRuns], fl[kMaxRuns];
+    // 2D foams have one z-row: 6 runs
+    constexpr int kRuns = 3 == D ? kMaxRuns : kMaxRuns / 3;
+    int off[kRuns], pre[kRuns], fl[kRuns];
     int total = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < (3 == D ? 3 : 1); ++a) {
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const bool row_ok = a < Z.n && b < Y.n;
@@ -432,8 +434,8 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                 const bool ok = row_ok && (0 == part || xr.two);
                 const int js = ok ? start[row + (0 == part ? xr.a0 : xr.a1)] : 0;
                 const int je = ok ? start[row + (0 == part ? xr.b0 : xr.b1) + 1] : 0;
-                j0[s] = js;
-                len[s] = je - js;
+                // candidate p of the run is bubble p + off
+                off[s] = js - total;
                 pre[s] = total;
                 fl[s] = fyz | (0 == part ? fx0 : fx1);
                 total += je - js;
@@ -447,16 +449,17 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
         int cnt = 0;
         for (int p0 = 0; p0 < total; p0 += 64) {
             const int p = p0 + lane;
-            int j = -1, f = 0;
+            // the run of candidate p is the last one starting at or before
+            // p (empty runs start where the next one does and are skipped)
+            int o = off[0], f = fl[0];
 #pragma unroll
-            for (int s = 0; s < kMaxRuns; ++s) {
-                const int o = p - pre[s];
-                if (o >= 0 && o < len[s]) {
-                    j = j0[s] + o;
-                    f = fl[s];
-                }
+            for (int s = 1; s < kRuns; ++s) {
+                const bool in = p >= pre[s];
+                o = in ? off[s] : o;
+                f = in ? fl[s] : f;
             }
-            const bool valid = j >= 0;
+            const bool valid = p < total;
+            const int j = valid ? p + o : -1;
             const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
             const int jl = valid ? j : i0;
             const D4 pj = D4{x4[jl].x, x4[jl].y, 3 == D ? x4[jl].z : 0.0, x4[jl].r};
