@@ -192,15 +192,27 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *cmem(const
     return (const __attribute__((address_space(4))) T *)p;
 }
 
-template <typename Body>
+// One neighbour record: the 32-byte FP64 record as two 16-byte loads, the
+// 16-byte fixed-point record as one.
+__device__ __forceinline__ D4 load_rec(const __attribute__((address_space(1))) D4 *p) {
+    return D4{p->x, p->y, p->z, p->r};
+}
+
+__device__ __forceinline__ Q4 load_rec(const __attribute__((address_space(1))) Q4 *p) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i v = *(const __attribute__((address_space(1))) v4i *)p;
+    return Q4{v.x, v.y, v.z, __int_as_float(v.w)};
+}
+
+template <typename Rec, typename Body>
 __device__ __forceinline__ void for_neighbours(const int *__restrict__ nbr_,
-                                               const D4 *__restrict__ p4_, size_t cap, int i,
+                                               const Rec *__restrict__ p4_, size_t cap, int i,
                                                int m_nn, Body &&body) {
     const auto nbr = gmem(nbr_);
     const auto p4 = gmem(p4_);
     for (int q0 = 0; q0 < m_nn; q0 += kGather) {
         int js[kGather];
-        D4 pj[kGather];
+        Rec pj[kGather];
         // every load is issued (past the end: the last entry, in bounds), then
         // the entries past the end are masked off
 #pragma unroll
@@ -212,12 +224,42 @@ __device__ __forceinline__ void for_neighbours(const int *__restrict__ nbr_,
 #pragma unroll
         for (int u = 0; u < kGather; ++u) {
             const int j = js[u] >= 0 ? js[u] : i;
-            pj[u] = D4{p4[j].x, p4[j].y, p4[j].z, p4[j].r};
+            pj[u] = load_rec(p4 + j);
         }
 #pragma unroll
         for (int u = 0; u < kGather; ++u)
             if (js[u] >= 0) body(js[u], pj[u]);
     }
+}
+
+// ---- fixed-point positions of the mixed-precision neighbour record ------
+// Code of coordinate x on axis (lb, L, wall): periodic axes take
+// round((x - lb) 2^32 / L) modulo 2^32, as a signed 32-bit value offset by
+// 2^31 (the offset cancels in differences); positions a little outside the
+// box wrap like the box does.  Wall axes take round((x - lb) 2^30 / L),
+// clamped to the int32 range.
+__device__ __forceinline__ int32_t fixed_code(double x, double lb, double L, bool wall) {
+    if (wall) {
+        double t = rint((x - lb) * (0x1p30 / L));
+        t = fmin(fmax(t, -2147483647.0), 2147483647.0);
+        return (int32_t)t;
+    }
+    double t = (x - lb) * (0x1p32 / L);
+    t -= 0x1p32 * floor(t * 0x1p-32);  // [0, 2^32)
+    t = rint(t - 0x1p31);               // [-2^31, 2^31]
+    if (t >= 0x1p31) t -= 0x1p32;
+    return (int32_t)t;
+}
+
+// Length of one code unit on an axis, in FP32.
+__device__ __forceinline__ float fixed_step(double L, bool wall) {
+    return (float)(L * (wall ? 0x1p-30 : 0x1p-32));
+}
+
+// Minimum-image (periodic) or plain (wall) difference a - b of two codes, in
+// code units: the wrapping 32-bit difference.
+__device__ __forceinline__ int fixed_diff(int32_t a, int32_t b) {
+    return (int)((uint32_t)a - (uint32_t)b);
 }
 
 // Squared length, x, y, z summed in order: reference src/vec.h:41-49.

@@ -65,6 +65,7 @@ size_t carve_foam(Foam &F, char *base) {
     F.r.idx_alt = L.take<int32_t>(cap);
     Scratch &S = F.s;
     S.p4 = L.take<D4>(cap);
+    S.q4 = CUBBLE_PREC_MIXED == F.precision ? L.take<Q4>(cap) : nullptr;
     S.x4 = L.take<D4>(cap);
     S.ov = L.take<double>(cap);
     S.del = L.take<uint8_t>(cap);
@@ -198,10 +199,12 @@ size_t DeviceArena::largest_free() const {
     return m;
 }
 
-size_t foam_footprint(int dim, int cap, int nbr_cap, int cell_cap, const Tuning &t) {
+size_t foam_footprint(int dim, int precision, int cap, int nbr_cap, int cell_cap,
+                      const Tuning &t) {
     Foam F;
     std::memset(&F, 0, sizeof(F));
     F.c.dim = dim;
+    F.precision = precision;
     size_foam(F, cap, nbr_cap > 0 ? nbr_cap : default_nbr_cap(dim), cell_cap, t);
     return carve_foam(F, nullptr) + align256((size_t)F.nbr_cap * cap * sizeof(int32_t));
 }

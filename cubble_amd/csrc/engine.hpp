@@ -71,7 +71,8 @@ struct FoamRec {
 };
 
 // Bytes of device memory one foam needs (the admission footprint).
-size_t foam_footprint(int dim, int cap, int nbr_cap, int cell_cap, const Tuning &t);
+size_t foam_footprint(int dim, int precision, int cap, int nbr_cap, int cell_cap,
+                      const Tuning &t);
 
 class Engine;
 // Copies of foam k's state between the device and caller-owned host views

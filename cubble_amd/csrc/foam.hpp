@@ -53,8 +53,19 @@ struct alignas(32) D4 {
     double x, y, z, r;
 };
 
+// Mixed precision only: one bubble's predicted position in 32-bit fixed
+// point over the box and its radius in FP32, 16 bytes, so a neighbour gather
+// is one 16-byte load.  Periodic axes map the box [lb, lb + L) onto the whole
+// 32-bit circle, so the wrapping difference of two codes is the minimum-image
+// difference; wall axes map [lb, lb + L] onto [0, 2^30].
+struct alignas(16) Q4 {
+    int32_t x, y, z;
+    float r;
+};
+
 struct Scratch {
     D4 *p4;              // predicted x, y, z, r of the current attempt (AoS)
+    Q4 *q4;              // the same in fixed point + FP32 (mixed precision; else null)
     D4 *x4;              // current x, y, z, r in cell order, written by the search
     double *ov;          // per-bubble overlap area sum
     uint8_t *del;        // deletion mark of the last attempt

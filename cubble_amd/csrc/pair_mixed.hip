@@ -1,15 +1,20 @@
 // pair_mixed.hip -- mixed-precision pairwise kernel.
 //
 // Same gather structure as the FP64 pair kernel (ascending neighbour order,
-// deterministic), but only the relative coordinate d = wrap(x_i - x_j) is
-// formed in FP64; the Durian velocity, overlap area and gas flux are then
-// evaluated in FP32 with hardware reciprocal square root / reciprocal, and
-// accumulated per bubble in FP32.  Positions, radii and everything the
-// corrector does stay FP64.  The overlap area uses the product form
+// deterministic), over a 16-byte neighbour record written by the predictor:
+// the predicted position in 32-bit fixed point over the box and the radius in
+// FP32 (one 16-byte load per neighbour instead of two).  The relative
+// coordinate d = wrap(x_i - x_j) is the exact integer difference of two codes
+// (the wrap is the integer wrap-around on periodic axes), rounded once to
+// FP32; a code unit is L / 2^32 (L / 2^30 on wall axes), so quantisation adds
+// at most L / 2^32 per component.  The Durian velocity, overlap area and gas
+// flux are evaluated in FP32 with hardware reciprocal square root /
+// reciprocal, and accumulated per bubble in FP32.  Positions, radii and
+// everything the corrector does stay FP64.  The overlap area uses the product
+// form
 //     a = (R^2 - |d|^2) (|d|^2 - (r1 - r2)^2) / (4 |d|^2),
 // algebraically equal to the reference's r2^2 - ((r2^2 - r1^2 + |d|^2)/2)^2/|d|^2
-// (src/kernels.cu:240-249) but without its cancellation near tangency; the
-// factor R^2 - |d|^2 is formed in FP64 before rounding to FP32.
+// (src/kernels.cu:240-249) but without its cancellation near tangency.
 #include "dev_util.hpp"
 #include "launch.hpp"
 
@@ -19,47 +24,38 @@ template <int D, bool GAS>
 __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R,
                                                  const Scratch &S, size_t cap, int base, int ipt,
                                                  int n, double &soa, double &soar) {
-    const D4 *__restrict__ p4 = S.p4;
+    const Q4 *__restrict__ q4 = S.q4;
     const int *__restrict__ nnA = S.nn;
     const int *__restrict__ nbr = S.nbr;
     const float f0 = (float)c.f0;
     const float pi_f = (float)kPi;
-    double half[3], L[3];
-    bool per[3];
+    float step[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        L[a] = c.interval[a];
-        half[a] = 0.5 * c.interval[a];
-        per[a] = !c.wall[a];
-    }
+    for (int a = 0; a < 3; ++a) step[a] = fixed_step(c.interval[a], c.wall[a] != 0);
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const D4 pi = p4[i];
-        const double xi = pi.x, yi = pi.y, zi = pi.z, ri = pi.r;
-        const float ri_f = (float)ri;
+        const Q4 qi = load_rec(gmem(q4) + i);
+        const float ri_f = qi.r;
         const float inv_ri = __frcp_rn(ri_f);
         float acc[3] = {0.f, 0.f, 0.f};
         float ovl = 0.f, dr = 0.f;
         const int m_nn = nnA[i];
-        for_neighbours(nbr, p4, cap, i, m_nn, [&](int j, const D4 &pj) {
-            const double rj = pj.r;
-            double dd[3] = {xi - pj.x, yi - pj.y, D == 3 ? zi - pj.z : 0.0};
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                if (per[a] && fabs(dd[a]) > half[a]) dd[a] -= dd[a] < 0 ? -L[a] : L[a];
-            }
-            const double m64 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
-            const double rad64 = ri + rj;
-            const double over64 = rad64 * rad64 - m64;
-            if (over64 >= 0.0) {
-                const float m = (float)m64;
+        for_neighbours(nbr, q4, cap, i, m_nn, [&](int j, const Q4 &qj) {
+            const float rj_f = qj.r;
+            // code differences are exact integers; one rounding to FP32 each
+            float dd[3] = {(float)fixed_diff(qi.x, qj.x) * step[0],
+                           (float)fixed_diff(qi.y, qj.y) * step[1],
+                           D == 3 ? (float)fixed_diff(qi.z, qj.z) * step[2] : 0.f};
+            const float m = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+            const float rad = ri_f + rj_f;
+            const float over = rad * rad - m;
+            if (over >= 0.f) {
                 const float rs = __frsqrt_rn(m);
-                const float s = rs - __frcp_rn((float)rad64);
+                const float s = rs - __frcp_rn(rad);
 #pragma unroll
-                for (int a = 0; a < D; ++a) acc[a] += (float)dd[a] * (f0 * s);
+                for (int a = 0; a < D; ++a) acc[a] += dd[a] * (f0 * s);
                 if (GAS) {
-                    const float rj_f = (float)rj;
                     const float r1 = i < j ? ri_f : rj_f;
                     const float r2 = i < j ? rj_f : ri_f;
                     const float r1sq = r1 * r1, r2sq = r2 * r2;
@@ -68,7 +64,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
                         a2 = r1sq < r2sq ? r1sq : r2sq;
                     } else {
                         const float dl = r1 - r2;
-                        a2 = (float)over64 * (m - dl * dl) * (0.25f * rs * rs);
+                        a2 = over * (m - dl * dl) * (0.25f * rs * rs);
                         a2 = a2 < 0.f ? -a2 : a2;
                     }
                     const float oa = 3 == D ? a2 * pi_f : 2.0f * __fsqrt_rn(a2);
@@ -82,7 +78,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
         R.vp[3][i] = (double)dr;
         S.ov[i] = (double)ovl;
         soa += (double)ovl;
-        soar += (double)ovl * (1.0 / ri);
+        soar += (double)ovl * (1.0 / (double)ri_f);
     }
 }
 

@@ -118,8 +118,8 @@ int admit(cubble_server *s, double t, bool &changed) {
             }
             q.drawn = true;
         }
-        const size_t need = foam_footprint(q.cfg.dim, q.cfg.num_start, q.cfg.nbr_cap,
-                                           q.io.cell_cap, tune);
+        const size_t need = foam_footprint(q.cfg.dim, q.cfg.precision, q.cfg.num_start,
+                                           q.cfg.nbr_cap, q.io.cell_cap, tune);
         // head-of-line: wait for memory; a request that can never fit fails
         if (need > e.arena.capacity()) {
             q.r.status = CUBBLE_E_NOMEM;
@@ -155,7 +155,8 @@ int64_t cubble_foam_footprint(const cubble_config *cfg) {
     InitOut io;
     std::string err;
     if (generate_foam(*cfg, x4, io, err)) return -1;
-    return (int64_t)foam_footprint(cfg->dim, cfg->num_start, cfg->nbr_cap, io.cell_cap, Tuning());
+    return (int64_t)foam_footprint(cfg->dim, cfg->precision, cfg->num_start, cfg->nbr_cap,
+                                   io.cell_cap, Tuning());
 }
 
 int cubble_server_create(const cubble_server_opts *opts, cubble_server **out) {

@@ -1,6 +1,7 @@
 // step_kernels.hip -- the three kernels of one step() attempt.
 //
-//   k_predict : AB2 predictor + block partials of S_A, S_Ar
+//   k_predict : AB2 predictor + block partials of S_A, S_Ar; in mixed
+//               precision also the 16-byte fixed-point neighbour record
 //               (reference preIntegrate, src/kernels.cu:40-170)
 //   k_pair    : pairwise Durian velocity, overlap area and gas flux, gathered
 //               per bubble over its full neighbour list; the last block
@@ -23,9 +24,10 @@
 namespace cubble_amd {
 
 // ---------------------------------------------------------------- predictor
-template <int D, bool GAS>
-__device__ __forceinline__ void predict_range(const Roles &R, D4 *__restrict__ p4, int base,
-                                              int ipt, int n, double h, double &sa, double &sar) {
+template <int D, bool GAS, bool FIXED>
+__device__ __forceinline__ void predict_range(const Consts &c, const Roles &R, D4 *__restrict__ p4,
+                                              Q4 *__restrict__ q4, int base, int ipt, int n,
+                                              double h, double &sa, double &sar) {
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
@@ -43,7 +45,22 @@ __device__ __forceinline__ void predict_range(const Roles &R, D4 *__restrict__ p
             rp = R.x[3][i];
         }
         p4[i] = D4{p[0], p[1], p[2], rp};
+        if (FIXED) {
+            int32_t qc[3] = {0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < D; ++q) qc[q] = fixed_code(p[q], c.lbb[q], c.interval[q], c.wall[q] != 0);
+            q4[i] = Q4{qc[0], qc[1], qc[2], (float)rp};
+        }
     }
+}
+
+template <int D, bool GAS>
+__device__ __forceinline__ void predict_dispatch(const Consts &c, const Roles &R,
+                                                 D4 *__restrict__ p4, Q4 *__restrict__ q4,
+                                                 int base, int ipt, int n, double h, double &sa,
+                                                 double &sar) {
+    if (q4) predict_range<D, GAS, true>(c, R, p4, q4, base, ipt, n, h, sa, sar);
+    else predict_range<D, GAS, false>(c, R, p4, q4, base, ipt, n, h, sa, sar);
 }
 
 __global__ __launch_bounds__(kBlock) void k_predict(Foam *foams, const int2 *bt, uint32_t flags) {
@@ -57,12 +74,14 @@ __global__ __launch_bounds__(kBlock) void k_predict(Foam *foams, const int2 *bt,
     double sa = 0.0, sar = 0.0;
     const bool gas = (flags & kGas) != 0;
     D4 *p4 = F->s.p4;
-    if (3 == F->c.dim) {
-        if (gas) predict_range<3, true>(R, p4, base, ipt, n, h, sa, sar);
-        else predict_range<3, false>(R, p4, base, ipt, n, h, sa, sar);
+    Q4 *q4 = F->s.q4;
+    const Consts c = F->c;
+    if (3 == c.dim) {
+        if (gas) predict_dispatch<3, true>(c, R, p4, q4, base, ipt, n, h, sa, sar);
+        else predict_dispatch<3, false>(c, R, p4, q4, base, ipt, n, h, sa, sar);
     } else {
-        if (gas) predict_range<2, true>(R, p4, base, ipt, n, h, sa, sar);
-        else predict_range<2, false>(R, p4, base, ipt, n, h, sa, sar);
+        if (gas) predict_dispatch<2, true>(c, R, p4, q4, base, ipt, n, h, sa, sar);
+        else predict_dispatch<2, false>(c, R, p4, q4, base, ipt, n, h, sa, sar);
     }
     sa = block_sum(sa);
     sar = block_sum(sar);

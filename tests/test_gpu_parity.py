@@ -196,9 +196,37 @@ def test_device_loop_equals_host_loop(dim, n):
     b.close()
 
 
-@pytest.mark.parametrize("dim,n,seed", [(2, 5000, 31), (3, 8000, 32), (3, 60000, 33)])
-def test_mixed_precision_attempt_error(ora, dim, n, seed):
-    cfg = config(dim, n, seed=seed, precision=ca.PREC_MIXED)
+# walls on some axes: the minimum image applies only to the periodic ones
+WALL_CASES = [(2, 4000, 61, (1, 0, 0)), (3, 6000, 62, (1, 0, 1)), (3, 5000, 63, (1, 1, 1))]
+
+
+@pytest.mark.parametrize("dim,n,seed,wall", WALL_CASES)
+def test_walls_first_search_and_attempt_fp64(ora, dim, n, seed, wall):
+    cfg = config(dim, n, seed=seed, wall=wall)
+    sim = ca.Simulator(cfg)
+    f = oracle_for(ora, cfg)
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    assert list(gsc["cell_dim"]) == list(osc["cell_dim"])
+    assert_same_structure(gst, ost)
+    assert gsc["num_pairs"] == osc["num_pairs"]
+    for _ in range(3):
+        sim.integrate(GAS)
+    f = oracle_from_sim(ora, sim)
+    sim.step(GAS)
+    f.attempt(True, True)
+    errs = attempt_errors(dim, sim.get_attempt(), f.attempt_out(), sim.scalars()["f0"])
+    print("wall fp64 errors", dim, n, wall, errs, flush=True)
+    for q, e in errs.items():
+        assert e <= FP64_TOL, (q, e)
+    sim.close()
+
+
+@pytest.mark.parametrize("dim,n,seed,wall", [(2, 5000, 31, (0, 0, 0)), (3, 8000, 32, (0, 0, 0)),
+                                             (3, 60000, 33, (0, 0, 0)), (2, 4000, 34, (0, 1, 0)),
+                                             (3, 6000, 35, (1, 0, 1))])
+def test_mixed_precision_attempt_error(ora, dim, n, seed, wall):
+    cfg = config(dim, n, seed=seed, precision=ca.PREC_MIXED, wall=wall)
     sim = ca.Simulator(cfg)
     for _ in range(3):
         sim.integrate(GAS)
