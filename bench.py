@@ -43,18 +43,22 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(dim, p):
+def kernel_bytes(dim, p, mixed=False):
     """Algorithmic (compulsory) HBM bytes per bubble of each kernel of an
     attempt; p = half-list pairs per bubble (each bubble reads 2p list
-    entries).  State is FP64 in both precision modes."""
+    entries).  State is FP64 in both precision modes; mixed precision adds the
+    16-byte fixed-point neighbour record, written by the predictor and the only
+    record the pair kernel reads."""
     c = dim + 1
+    rec = 16 if mixed else 8 * c
     return {
-        # read x, r, v, v_old; write the predicted {x, y, z, r} record
-        "predict": 8 * 4 * c,
+        # read x, r, v, v_old; write the predicted {x, y, z, r} record (and in
+        # mixed precision the fixed-point record)
+        "predict": 8 * 4 * c + (16 if mixed else 0),
         # read own record, neighbour count, 2p neighbour indices (the gathered
         # neighbour records are other bubbles' own records: L2 hits when the
         # lists are cell ordered); write v_p incl. the pair dR/dt, overlap
-        "pair": 8 * c + 4 + 8 * p + 8 * c + 8,
+        "pair": rec + 4 + 8 * p + 8 * c + 8,
         # read predicted record, x, v, v_p, saved (c each), overlap, path, error;
         # write corrected x, dR/dt, path, error (+1 byte deletion mark)
         "correct": 8 * (5 * c + 3) + 8 * (c + 3) + 1,
@@ -84,7 +88,7 @@ def roofline(kt, per_foam, mixed=False):
     nbytes = 0.0
     bubbles = 0
     for dim, n, pairs in per_foam:
-        nbytes += kernel_bytes(dim, pairs / max(n, 1))[dom] * n
+        nbytes += kernel_bytes(dim, pairs / max(n, 1), mixed)[dom] * n
         bubbles += n
     achieved = nbytes / (times[dom] * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -320,11 +324,16 @@ def read_counters(folder, counter):
     return out
 
 
-def measure_traffic(kernel):
-    """HBM bytes per launch of `kernel` over the headline workload, from two
-    rocprofv3 --pmc passes over a child run of this script (FETCH_SIZE, then
-    WRITE_SIZE; KiB per dispatch), each scaled by what the same counter reads
-    for the copy of COPY_MIB MiB.  Run before this process touches the GPU."""
+ATTEMPT_KERNELS = ("k_predict", "k_pair_mixed", "k_correct")
+
+
+def measure_traffic(kernels=ATTEMPT_KERNELS):
+    """HBM bytes per launch of each of `kernels` over the headline workload,
+    from two rocprofv3 --pmc passes over a child run of this script
+    (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
+    same counter reads for the copy of COPY_MIB MiB.  Returns ({kernel: bytes},
+    {kernel: counter details}) or (None, reason).  Run before this process
+    touches the GPU."""
     import shutil
     import subprocess
     import tempfile
@@ -333,7 +342,7 @@ def measure_traffic(kernel):
     if not os.path.exists(exe):
         return None, "rocprofv3 not found"
     env = dict(os.environ, TMPDIR="/tmp")
-    res = {}
+    res = {k: {} for k in kernels}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         folder = tempfile.mkdtemp(prefix="cubble_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", folder, "-o", "pmc", "--",
@@ -347,16 +356,19 @@ def measure_traffic(kernel):
         vals = read_counters(folder, counter)
         shutil.rmtree(folder, ignore_errors=True)
         copy = [v for k, v in vals.items() if "copy" in k.lower()]
-        mine = vals.get(kernel)
-        if r.returncode not in (0, -11, 139) and not mine:
+        missing = [k for k in kernels if not vals.get(k)]
+        if r.returncode not in (0, -11, 139) and missing:
             return None, "rocprofv3 %s pass failed (rc %d)" % (counter, r.returncode)
-        if not copy or not mine:
-            return None, "%s pass: no reading for %s or the copy kernel" % (counter, kernel)
+        if not copy or missing:
+            return None, "%s pass: no reading for %s or the copy kernel" % (counter, missing)
         copy_kib = max(max(c) for c in copy)
         scale = (COPY_MIB << 20) / (copy_kib * 1024.0)
-        res[counter] = {"kib_per_dispatch": sum(mine) / len(mine), "copy_kib": copy_kib,
-                        "scale": scale}
-    total = sum(v["kib_per_dispatch"] * 1024.0 * v["scale"] for v in res.values())
+        for k in kernels:
+            mine = vals[k]
+            res[k][counter] = {"kib_per_dispatch": sum(mine) / len(mine), "copy_kib": copy_kib,
+                               "scale": scale}
+    total = {k: sum(v["kib_per_dispatch"] * 1024.0 * v["scale"] for v in res[k].values())
+             for k in kernels}
     return total, res
 
 
@@ -428,7 +440,7 @@ def main():
     traffic = (None, "not measured")
     if "batch" in legs and not args.no_traffic:
         # counter passes first: this process has not touched the GPU yet
-        traffic = measure_traffic("k_pair_mixed")
+        traffic = measure_traffic()
         log("traffic: %s" % json.dumps(traffic))
     out = {"metric": METRIC, "unit": "bubble-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
@@ -453,12 +465,13 @@ def main():
                                "counters": b["counters"]},
                     "roofline": b["roofline"]})
         rf = out["roofline"]
-        if rf["kernel"] == "k_pair_mixed" and traffic[0] is not None:
-            rf["traffic"] = round(traffic[0])
-            rf["traffic_counters"] = traffic[1]
+        if traffic[0] is not None and rf["kernel"] in traffic[0]:
+            rf["traffic"] = round(traffic[0][rf["kernel"]])
+            rf["traffic_counters"] = traffic[1][rf["kernel"]]
+            rf["traffic_all_attempt_kernels"] = {k: round(v) for k, v in traffic[0].items()}
         else:
             rf["traffic_note"] = traffic[1] if traffic[0] is None else \
-                "counters taken for k_pair_mixed, not the dominant kernel"
+                "no counters for %s" % rf["kernel"]
     if "plain" in legs:
         pl = leg_plain(ca, args, ip, cfgs)
         out["plain_fp64_same_foams"] = pl
