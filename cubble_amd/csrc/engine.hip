@@ -64,8 +64,11 @@ size_t carve_foam(Foam &F, char *base) {
     F.r.idx = L.take<int32_t>(cap);
     F.r.idx_alt = L.take<int32_t>(cap);
     Scratch &S = F.s;
-    S.p4 = L.take<D4>(cap);
-    S.q4 = CUBBLE_PREC_MIXED == F.precision ? L.take<Q4>(cap) : nullptr;
+    // FP64: the 32-byte predicted record; mixed precision: the 16-byte one
+    // (k_correct recomputes the FP64 prediction)
+    const bool mixed = CUBBLE_PREC_MIXED == F.precision;
+    S.p4 = mixed ? nullptr : L.take<D4>(cap);
+    S.q4 = mixed ? L.take<Q4>(cap) : nullptr;
     S.x4 = L.take<D4>(cap);
     S.ov = L.take<double>(cap);
     S.del = L.take<uint8_t>(cap);

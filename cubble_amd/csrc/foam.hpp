@@ -64,7 +64,7 @@ struct alignas(16) Q4 {
 };
 
 struct Scratch {
-    D4 *p4;              // predicted x, y, z, r of the current attempt (AoS)
+    D4 *p4;              // predicted x, y, z, r of the current attempt (FP64 mode; else null)
     Q4 *q4;              // the same in fixed point + FP32 (mixed precision; else null)
     D4 *x4;              // current x, y, z, r in cell order, written by the search
     double *ov;          // per-bubble overlap area sum

@@ -1,8 +1,9 @@
 // step_kernels.hip -- the three kernels of one step() attempt.
 //
-//   k_predict : AB2 predictor + block partials of S_A, S_Ar; in mixed
-//               precision also the 16-byte fixed-point neighbour record
-//               (reference preIntegrate, src/kernels.cu:40-170)
+//   k_predict : AB2 predictor + block partials of S_A, S_Ar; writes the
+//               32-byte FP64 neighbour record, or in mixed precision the
+//               16-byte fixed-point one (reference preIntegrate,
+//               src/kernels.cu:40-170)
 //   k_pair    : pairwise Durian velocity, overlap area and gas flux, gathered
 //               per bubble over its full neighbour list; the last block
 //               reduces S_A, S_Ar, S_OA, S_OAr
@@ -18,12 +19,28 @@
 // unspecified order.  Here each bubble gathers its own neighbours in
 // ascending index order, so results are deterministic, independent of the
 // launch shape, and need no atomics.  All reductions are fixed-order trees.
+#include <type_traits>
+
 #include "dev_util.hpp"
 #include "launch.hpp"
 
 namespace cubble_amd {
 
 // ---------------------------------------------------------------- predictor
+// AB2 predicted position and radius of bubble i: reference
+// src/kernels.cu:48-50.  Shared by k_predict and, in mixed precision (where
+// no FP64 record is stored), by k_correct, so both round identically.
+template <int D, bool GAS>
+__device__ __forceinline__ D4 predicted(const Roles &R, double h, int i) {
+    double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < D; ++q) p[q] = R.x[q][i] + h * (3.0 * R.v[q][i] - R.vo[q][i]);
+    const double rp = GAS ? R.x[3][i] + h * (3.0 * R.v[3][i] - R.vo[3][i]) : R.x[3][i];
+    return D4{p[0], p[1], p[2], rp};
+}
+
+// FP64: writes the 32-byte record the FP64 pair kernel gathers.  Mixed
+// (FIXED): writes only the 16-byte fixed-point record.
 template <int D, bool GAS, bool FIXED>
 __device__ __forceinline__ void predict_range(const Consts &c, const Roles &R, D4 *__restrict__ p4,
                                               Q4 *__restrict__ q4, int base, int ipt, int n,
@@ -31,25 +48,21 @@ __device__ __forceinline__ void predict_range(const Consts &c, const Roles &R, D
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        double p[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int q = 0; q < D; ++q) p[q] = R.x[q][i] + h * (3.0 * R.v[q][i] - R.vo[q][i]);
-        double rp;
+        const D4 pr = predicted<D, GAS>(R, h, i);
         if (GAS) {
-            rp = R.x[3][i] + h * (3.0 * R.v[3][i] - R.vo[3][i]);
             double apr = 2.0 * kPi;
-            if (3 == D) apr *= 2.0 * rp;
-            sa += apr * rp;
+            if (3 == D) apr *= 2.0 * pr.r;
+            sa += apr * pr.r;
             sar += apr;
-        } else {
-            rp = R.x[3][i];
         }
-        p4[i] = D4{p[0], p[1], p[2], rp};
         if (FIXED) {
+            const double p[3] = {pr.x, pr.y, pr.z};
             int32_t qc[3] = {0, 0, 0};
 #pragma unroll
             for (int q = 0; q < D; ++q) qc[q] = fixed_code(p[q], c.lbb[q], c.interval[q], c.wall[q] != 0);
-            q4[i] = Q4{qc[0], qc[1], qc[2], (float)rp};
+            q4[i] = Q4{qc[0], qc[1], qc[2], (float)pr.r};
+        } else {
+            p4[i] = pr;
         }
     }
 }
@@ -261,7 +274,8 @@ struct CorrAcc {
     int n_del;
 };
 
-template <int D, bool GAS, bool PATH>
+// MIXED: no FP64 predicted record is stored; recompute it.
+template <int D, bool GAS, bool PATH, bool MIXED>
 __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, const Scratch &S,
                                               const Dyn &dyn, int base, int ipt, int n,
                                               CorrAcc &a) {
@@ -271,7 +285,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const D4 pp = S.p4[i];
+        const D4 pp = MIXED ? predicted<D, GAS>(R, h, i) : S.p4[i];
         const double pv[3] = {pp.x, pp.y, pp.z};
         double vp[3];
 #pragma unroll
@@ -402,13 +416,18 @@ __device__ __forceinline__ void correct_dispatch(const Consts &c, const Roles &R
                                                  const Dyn &dyn, int base, int ipt, int n,
                                                  uint32_t flags, CorrAcc &a) {
     const bool gas = (flags & kGas) != 0, path = (flags & kPath) != 0;
-    if (gas) {
-        if (path) correct_range<D, true, true>(c, R, S, dyn, base, ipt, n, a);
-        else correct_range<D, true, false>(c, R, S, dyn, base, ipt, n, a);
-    } else {
-        if (path) correct_range<D, false, true>(c, R, S, dyn, base, ipt, n, a);
-        else correct_range<D, false, false>(c, R, S, dyn, base, ipt, n, a);
-    }
+    auto run = [&](auto mixed) {
+        constexpr bool M = decltype(mixed)::value;
+        if (gas) {
+            if (path) correct_range<D, true, true, M>(c, R, S, dyn, base, ipt, n, a);
+            else correct_range<D, true, false, M>(c, R, S, dyn, base, ipt, n, a);
+        } else {
+            if (path) correct_range<D, false, true, M>(c, R, S, dyn, base, ipt, n, a);
+            else correct_range<D, false, false, M>(c, R, S, dyn, base, ipt, n, a);
+        }
+    };
+    if (S.q4) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt, uint32_t flags) {
