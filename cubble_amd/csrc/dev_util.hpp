@@ -16,6 +16,12 @@
 
 namespace cubble_amd {
 
+// Velocity buffer as its element type: double (FP64 mode) or float (mixed).
+template <typename V>
+__device__ __forceinline__ V *vel(void *p) {
+    return static_cast<V *>(p);
+}
+
 struct Ctx {
     Foam *F;
     int lb;  // logical block within the foam

@@ -37,13 +37,17 @@ size_t carve_foam(Foam &F, char *base) {
     Carver L{base};
     const size_t cap = (size_t)F.cap;
     const int D = F.c.dim;
-    double *xb[2][4] = {}, *vb[4][4] = {};
+    double *xb[2][4] = {};
+    void *vb[4][4] = {};
+    // velocities: FP32 in mixed precision
+    const bool fvel = CUBBLE_PREC_MIXED == F.precision;
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < 4; ++q)
             if (!(2 == q && 2 == D)) xb[a][q] = L.take<double>(cap);
     for (int a = 0; a < 4; ++a)
         for (int q = 0; q < 4; ++q)
-            if (!(2 == q && 2 == D)) vb[a][q] = L.take<double>(cap);
+            if (!(2 == q && 2 == D))
+                vb[a][q] = fvel ? (void *)L.take<float>(cap) : (void *)L.take<double>(cap);
     for (int q = 0; q < 4; ++q) {
         F.r.x[q] = xb[0][q];
         F.r.xp[q] = xb[1][q];

@@ -14,6 +14,8 @@
 
 #include <stdint.h>
 
+#include "cubble_amd.h"  // CUBBLE_PREC_* (Foam::precision)
+
 namespace cubble_amd {
 
 constexpr int kBlock = 256;         // threads per block, bubble- and cell-space
@@ -33,13 +35,16 @@ struct Consts {
 };
 
 // Role pointers: which buffer currently plays each part of the state.
+// Velocity buffers hold FP64 in FP64 mode and FP32 in mixed precision, where
+// every velocity is an FP32 result of the pair kernel (plus the corrector's
+// wall and gas terms, rounded once); kernels read them through vel<V>().
 struct Roles {
     double *x[4];        // current x, y, z, r
     double *xp[4];       // predicted, then corrected (attempt output)
-    double *v[4];        // current velocities
-    double *vp[4];       // predicted velocities (attempt output)
-    double *vo[4];       // old velocities
-    double *vs[4];       // spare (search reorder target)
+    void *v[4];          // current velocities
+    void *vp[4];         // predicted velocities (attempt output)
+    void *vo[4];         // old velocities
+    void *vs[4];         // spare (search reorder target)
     double *saved[4];    // positions at the last search
     double *path, *path_new;
     double *err, *err_alt;

@@ -9,8 +9,9 @@
 // FP32; a code unit is L / 2^32 (L / 2^30 on wall axes), so quantisation adds
 // at most L / 2^32 per component.  The Durian velocity, overlap area and gas
 // flux are evaluated in FP32 with hardware reciprocal square root /
-// reciprocal, and accumulated per bubble in FP32.  Positions, radii and
-// everything the corrector does stay FP64.  The overlap area uses the product
+// reciprocal, and accumulated per bubble in FP32, and stored as FP32 (the
+// velocity buffers of mixed precision).  Positions, radii and the corrector's
+// arithmetic stay FP64.  The overlap area uses the product
 // form
 //     a = (R^2 - |d|^2) (|d|^2 - (r1 - r2)^2) / (4 |d|^2),
 // algebraically equal to the reference's r2^2 - ((r2^2 - r1^2 + |d|^2)/2)^2/|d|^2
@@ -74,8 +75,8 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
             }
         });
 #pragma unroll
-        for (int a = 0; a < D; ++a) R.vp[a][i] = (double)acc[a];
-        R.vp[3][i] = (double)dr;
+        for (int a = 0; a < D; ++a) vel<float>(R.vp[a])[i] = acc[a];
+        vel<float>(R.vp[3])[i] = dr;
         S.ov[i] = (double)ovl;
         soa += (double)ovl;
         soar += (double)ovl * (1.0 / (double)ri_f);

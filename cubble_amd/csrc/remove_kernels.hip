@@ -69,7 +69,7 @@ __device__ __forceinline__ void mark_block(Foam *F, int lb) {
     }
 }
 
-template <int D>
+template <int D, typename V>
 __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, size_t cap, int base,
                                            int ipt, int nnew, int h, double vm) {
     int sum_nn = 0;
@@ -91,8 +91,8 @@ __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, siz
             for (int q = 0; q < 4; ++q) {
                 if (2 == q && 2 == D) continue;
                 R.x[q][i] = R.x[q][src];
-                R.v[q][i] = R.v[q][src];
-                R.vo[q][i] = R.vo[q][src];
+                vel<V>(R.v[q])[i] = vel<V>(R.v[q])[src];
+                vel<V>(R.vo[q])[i] = vel<V>(R.vo[q])[src];
                 R.saved[q][i] = R.saved[q][src];
             }
             R.path[i] = R.path[src];
@@ -138,8 +138,13 @@ __device__ __forceinline__ void apply_block(Foam *F, int lb) {
     const double vm = F->d.vol_fix;
     const size_t cap = (size_t)F->cap;
     const int base = lb * kBlock * ipt;
-    int sum_nn = 3 == F->c.dim ? apply_range<3>(R, S, cap, base, ipt, nnew, h, vm)
-                               : apply_range<2>(R, S, cap, base, ipt, nnew, h, vm);
+    int sum_nn;
+    if (CUBBLE_PREC_MIXED == F->precision)  // FP32 velocities
+        sum_nn = 3 == F->c.dim ? apply_range<3, float>(R, S, cap, base, ipt, nnew, h, vm)
+                               : apply_range<2, float>(R, S, cap, base, ipt, nnew, h, vm);
+    else
+        sum_nn = 3 == F->c.dim ? apply_range<3, double>(R, S, cap, base, ipt, nnew, h, vm)
+                               : apply_range<2, double>(R, S, cap, base, ipt, nnew, h, vm);
     sum_nn = block_sum_i(sum_nn);
     if (threadIdx.x == 0) S.part_i[lb * kPartI + 1] = sum_nn;
 }

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_s_bin(Foam *foams, const int2 *bt) {
     }
 }
 
-template <int D>
+template <int D, typename V>
 __device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, int base, int ipt,
                                               int n) {
     for (int k = 0; k < ipt; ++k) {
@@ -161,8 +161,8 @@ __device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, 
             xs[q] = xv;
             R.xp[q][np] = xv;
             R.saved[q][np] = xv;
-            R.vp[q][np] = R.v[q][i];
-            R.vs[q][np] = R.vo[q][i];
+            vel<V>(R.vp[q])[np] = vel<V>(R.v[q])[i];
+            vel<V>(R.vs[q])[np] = vel<V>(R.vo[q])[i];
         }
         S.x4[np] = D4{xs[0], xs[1], xs[2], xs[3]};
         R.path_new[np] = R.path[i];
@@ -181,8 +181,14 @@ __global__ __launch_bounds__(kBlock) void k_s_reorder(Foam *foams, const int2 *b
     const Scratch S = F->s;
     const int n = F->d.count, ipt = F->ipt;
     const int base = cx.lb * kBlock * ipt;
-    if (3 == F->c.dim) reorder_range<3>(R, S, base, ipt, n);
-    else reorder_range<2>(R, S, base, ipt, n);
+    const bool fvel = CUBBLE_PREC_MIXED == F->precision;  // FP32 velocities
+    if (3 == F->c.dim) {
+        if (fvel) reorder_range<3, float>(R, S, base, ipt, n);
+        else reorder_range<3, double>(R, S, base, ipt, n);
+    } else {
+        if (fvel) reorder_range<2, float>(R, S, base, ipt, n);
+        else reorder_range<2, double>(R, S, base, ipt, n);
+    }
 }
 
 // After the reorder the new state sits in the alternate buffers; k_s_fin
@@ -192,12 +198,12 @@ __device__ void swap_reordered_roles(Roles &W) {
         double *t = W.x[q];
         W.x[q] = W.xp[q];
         W.xp[q] = t;
-        t = W.v[q];
+        void *u = W.v[q];
         W.v[q] = W.vp[q];
-        W.vp[q] = t;
-        t = W.vo[q];
+        W.vp[q] = u;
+        u = W.vo[q];
         W.vo[q] = W.vs[q];
-        W.vs[q] = t;
+        W.vs[q] = u;
     }
     double *t = W.path;
     W.path = W.path_new;

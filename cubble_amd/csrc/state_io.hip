@@ -86,6 +86,26 @@ int h2d(Engine &e, void *dst, const void *src, size_t bytes) {
     return CUBBLE_OK;
 }
 
+// Velocity buffers are FP32 in mixed precision, FP64 otherwise; host views
+// are FP64 in both.
+int d2h_vel(Engine &e, const Foam &f, double *dst, const void *src, size_t n) {
+    if (!dst || !n) return CUBBLE_OK;
+    if (CUBBLE_PREC_MIXED != f.precision) return d2h(e, dst, src, n * sizeof(double));
+    std::vector<float> t(n);
+    int rc = d2h(e, t.data(), src, n * sizeof(float));
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) dst[i] = (double)t[i];
+    return CUBBLE_OK;
+}
+
+int h2d_vel(Engine &e, const Foam &f, void *dst, const double *src, size_t n) {
+    if (!src || !n) return CUBBLE_OK;
+    if (CUBBLE_PREC_MIXED != f.precision) return h2d(e, dst, src, n * sizeof(double));
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; ++i) t[i] = (float)src[i];
+    return h2d(e, dst, t.data(), n * sizeof(float));
+}
+
 }  // namespace
 
 int get_state(Engine &e, int k, cubble_state_view *v) {
@@ -101,8 +121,9 @@ int get_state(Engine &e, int k, cubble_state_view *v) {
                 if (p) std::memset(p, 0, db);
             continue;
         }
-        if ((rc = d2h(e, v->x[q], f.r.x[q], db)) || (rc = d2h(e, v->v[q], f.r.v[q], db)) ||
-            (rc = d2h(e, v->vo[q], f.r.vo[q], db)) || (rc = d2h(e, v->saved[q], f.r.saved[q], db)))
+        if ((rc = d2h(e, v->x[q], f.r.x[q], db)) || (rc = d2h_vel(e, f, v->v[q], f.r.v[q], n)) ||
+            (rc = d2h_vel(e, f, v->vo[q], f.r.vo[q], n)) ||
+            (rc = d2h(e, v->saved[q], f.r.saved[q], db)))
             return rc;
     }
     if ((rc = d2h(e, v->path, f.r.path, db)) || (rc = d2h(e, v->error, f.r.err, db))) return rc;
@@ -152,7 +173,7 @@ int get_attempt(Engine &e, int k, cubble_attempt_view *v) {
             if (v->vp[q]) std::memset(v->vp[q], 0, db);
             continue;
         }
-        if ((rc = d2h(e, v->xc[q], f.r.xp[q], db)) || (rc = d2h(e, v->vp[q], f.r.vp[q], db)))
+        if ((rc = d2h(e, v->xc[q], f.r.xp[q], db)) || (rc = d2h_vel(e, f, v->vp[q], f.r.vp[q], n)))
             return rc;
     }
     if ((rc = d2h(e, v->overlap, f.s.ov, db)) || (rc = d2h(e, v->path_new, f.r.path_new, db)) ||
@@ -172,8 +193,9 @@ int set_state(Engine &e, int k, const cubble_state_view *v, const cubble_scalars
     const size_t db = (size_t)n * sizeof(double), ib = (size_t)n * sizeof(int32_t);
     for (int q = 0; q < 4; ++q) {
         if (2 == q && 2 == f.c.dim) continue;
-        if ((rc = h2d(e, f.r.x[q], v->x[q], db)) || (rc = h2d(e, f.r.v[q], v->v[q], db)) ||
-            (rc = h2d(e, f.r.vo[q], v->vo[q], db)) || (rc = h2d(e, f.r.saved[q], v->saved[q], db)))
+        if ((rc = h2d(e, f.r.x[q], v->x[q], db)) || (rc = h2d_vel(e, f, f.r.v[q], v->v[q], n)) ||
+            (rc = h2d_vel(e, f, f.r.vo[q], v->vo[q], n)) ||
+            (rc = h2d(e, f.r.saved[q], v->saved[q], db)))
             return rc;
     }
     if ((rc = h2d(e, f.r.path, v->path, db)) || (rc = h2d(e, f.r.err, v->error, db))) return rc;

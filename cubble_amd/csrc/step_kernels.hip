@@ -30,17 +30,20 @@ namespace cubble_amd {
 // AB2 predicted position and radius of bubble i: reference
 // src/kernels.cu:48-50.  Shared by k_predict and, in mixed precision (where
 // no FP64 record is stored), by k_correct, so both round identically.
-template <int D, bool GAS>
+template <int D, bool GAS, typename V>
 __device__ __forceinline__ D4 predicted(const Roles &R, double h, int i) {
     double p[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < D; ++q) p[q] = R.x[q][i] + h * (3.0 * R.v[q][i] - R.vo[q][i]);
-    const double rp = GAS ? R.x[3][i] + h * (3.0 * R.v[3][i] - R.vo[3][i]) : R.x[3][i];
+    for (int q = 0; q < D; ++q)
+        p[q] = R.x[q][i] + h * (3.0 * (double)vel<V>(R.v[q])[i] - (double)vel<V>(R.vo[q])[i]);
+    const double rp =
+        GAS ? R.x[3][i] + h * (3.0 * (double)vel<V>(R.v[3])[i] - (double)vel<V>(R.vo[3])[i])
+            : R.x[3][i];
     return D4{p[0], p[1], p[2], rp};
 }
 
 // FP64: writes the 32-byte record the FP64 pair kernel gathers.  Mixed
-// (FIXED): writes only the 16-byte fixed-point record.
+// (FIXED, FP32 velocities): writes only the 16-byte fixed-point record.
 template <int D, bool GAS, bool FIXED>
 __device__ __forceinline__ void predict_range(const Consts &c, const Roles &R, D4 *__restrict__ p4,
                                               Q4 *__restrict__ q4, int base, int ipt, int n,
@@ -48,7 +51,8 @@ __device__ __forceinline__ void predict_range(const Consts &c, const Roles &R, D
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const D4 pr = predicted<D, GAS>(R, h, i);
+        const D4 pr = predicted<D, GAS, typename std::conditional<FIXED, float, double>::type>(
+            R, h, i);
         if (GAS) {
             double apr = 2.0 * kPi;
             if (3 == D) apr *= 2.0 * pr.r;
@@ -158,8 +162,8 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
             }
         });
 #pragma unroll
-        for (int a = 0; a < D; ++a) R.vp[a][i] = acc[a];
-        R.vp[3][i] = dr;
+        for (int a = 0; a < D; ++a) vel<double>(R.vp[a])[i] = acc[a];
+        vel<double>(R.vp[3])[i] = dr;
         S.ov[i] = ovl;
         soa += ovl;
         soar += ovl * inv_ri;
@@ -274,28 +278,33 @@ struct CorrAcc {
     int n_del;
 };
 
-// MIXED: no FP64 predicted record is stored; recompute it.
+// MIXED: no FP64 predicted record is stored (recompute it) and velocities
+// are FP32.
 template <int D, bool GAS, bool PATH, bool MIXED>
 __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, const Scratch &S,
                                               const Dyn &dyn, int base, int ipt, int n,
                                               CorrAcc &a) {
+    typedef typename std::conditional<MIXED, float, double>::type V;
     const double h = 0.5 * dyn.dt;
     const bool walls = c.wall[0] || c.wall[1] || (3 == D && c.wall[2]);
     const double inv_rho = (dyn.s_ar - dyn.s_oar) / (dyn.s_a - dyn.s_oa);
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const D4 pp = MIXED ? predicted<D, GAS>(R, h, i) : S.p4[i];
+        const D4 pp = MIXED ? predicted<D, GAS, V>(R, h, i) : S.p4[i];
         const double pv[3] = {pp.x, pp.y, pp.z};
         double vp[3];
 #pragma unroll
-        for (int q = 0; q < D; ++q) vp[q] = R.vp[q][i];
+        for (int q = 0; q < D; ++q) vp[q] = (double)vel<V>(R.vp[q])[i];
         if (walls) {
             double vz = 3 == D ? vp[2] : 0.0;
             add_wall_velocity<D>(c, pp.x, pp.y, pp.z, pp.r, vp[0], vp[1], vz);
             if (3 == D) vp[2] = vz;
 #pragma unroll
-            for (int q = 0; q < D; ++q) R.vp[q][i] = vp[q];
+            for (int q = 0; q < D; ++q) {
+                vp[q] = (double)(V)vp[q];  // the stored value drives the corrector
+                vel<V>(R.vp[q])[i] = (V)vp[q];
+            }
         }
         double e = 0.0, dist = 0.0;
         double xo[3], xc[3];
@@ -303,7 +312,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
         for (int q = 0; q < D; ++q) {
             const double pred = pv[q];
             xo[q] = R.x[q][i];
-            const double corr = xo[q] + h * (R.v[q][i] + vp[q]);
+            const double corr = xo[q] + h * ((double)vel<V>(R.v[q])[i] + vp[q]);
             xc[q] = corr;
             R.xp[q][i] = corr;
             e = fmax(e, fabs(pred - corr));
@@ -318,11 +327,12 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             if (3 == D) area *= 2.0 * r;
             double vr = inv_rho - 1.0 / r;
             vr *= c.kappa * c.avg_area_in * n / dyn.s_a * (area - S.ov[i]);
-            vr += R.vp[3][i];
+            vr += (double)vel<V>(R.vp[3])[i];
             vr *= c.k_param / area;
-            R.vp[3][i] = vr;
+            vel<V>(R.vp[3])[i] = (V)vr;
+            vr = (double)(V)vr;  // the stored value drives the corrector
             const double pred = r;
-            const double corr = R.x[3][i] + h * (R.v[3][i] + vr);
+            const double corr = R.x[3][i] + h * ((double)vel<V>(R.v[3])[i] + vr);
             R.xp[3][i] = corr;
             e = fmax(e, fabs(pred - corr));
             dist += corr - R.saved[3][i];
@@ -384,13 +394,13 @@ __device__ void control(Foam *F, uint32_t flags, const CorrAcc &a) {
     for (int k = 0; k < 4; ++k) {
         if (2 == k && 2 == c.dim) continue;
         if (3 == k && !gas) continue;
-        double *t = R.vo[k];
+        void *t = R.vo[k];
         R.vo[k] = R.v[k];
         R.v[k] = R.vp[k];
         R.vp[k] = t;
-        t = R.x[k];
+        double *u = R.x[k];
         R.x[k] = R.xp[k];
-        R.xp[k] = t;
+        R.xp[k] = u;
     }
     if (flags & kPath) {
         double *t = R.path;
