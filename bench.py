@@ -46,23 +46,26 @@ def log(msg):
 def kernel_bytes(dim, p, mixed=False):
     """Algorithmic (compulsory) HBM bytes per bubble of each kernel of an
     attempt; p = half-list pairs per bubble (each bubble reads 2p list
-    entries).  State is FP64 in both precision modes; in mixed precision the
-    predictor writes the 16-byte fixed-point neighbour record instead of the
-    32-byte FP64 one, and the pair kernel reads only that."""
+    entries).  Positions and radii are FP64 in both precision modes; mixed
+    precision stores velocities in FP32, and its predictor writes the 16-byte
+    fixed-point neighbour record instead of the 32-byte FP64 one (the pair
+    kernel reads only that, the corrector recomputes the prediction)."""
     c = dim + 1
     rec = 16 if mixed else 8 * c
+    vb = 4 if mixed else 8  # bytes per velocity component (FP32 in mixed precision)
     return {
         # read x, r, v, v_old; write the predicted {x, y, z, r} record (in
         # mixed precision only the 16-byte fixed-point record)
-        "predict": 8 * 3 * c + rec,
+        "predict": 8 * c + 2 * vb * c + rec,
         # read own record, neighbour count, 2p neighbour indices (the gathered
         # neighbour records are other bubbles' own records: L2 hits when the
         # lists are cell ordered); write v_p incl. the pair dR/dt, overlap
-        "pair": rec + 4 + 8 * p + 8 * c + 8,
+        "pair": rec + 4 + 8 * p + vb * c + 8,
         # read predicted record (mixed precision: v_old, to recompute it), x, v,
         # v_p, saved (c each), overlap, path, error; write corrected x, dR/dt,
         # path, error (+1 byte deletion mark)
-        "correct": 8 * (5 * c + 3) + 8 * (c + 3) + 1,
+        "correct": (vb if mixed else 8) * c + 8 * c + 2 * vb * c + 8 * c + 24
+                   + 8 * c + vb + 17,
     }
 
 
