@@ -30,33 +30,67 @@
 
 namespace cubble_amd {
 
+// Runs of equal keys among the wave's valid lanes (bubbles are in the cell
+// order of the previous search, so neighbouring lanes mostly share a cell):
+// `heads` has a bit for the first lane of each run; a head's run ends at the
+// next head or at the last valid lane.  One atomic per run instead of one per
+// bubble.
+struct WaveRuns {
+    unsigned long long heads;
+    int len;   // run length (valid on head lanes)
+    int head;  // lane of this lane's run head
+};
+
+__device__ __forceinline__ WaveRuns wave_runs(int key, bool valid) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int prev = __shfl_up(key, 1);
+    const bool is_head = valid && (0 == lane || prev != key);
+    const unsigned long long heads = __builtin_amdgcn_ballot_w64(is_head);
+    const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
+    // valid lanes are a prefix of the wave (consecutive bubbles below n)
+    const int nvalid = __popcll(vmask);
+    const unsigned long long above = lane < 63 ? heads & (~0ull << (lane + 1)) : 0ull;
+    const int end = above ? __builtin_ctzll(above) : nvalid;
+    const unsigned long long upto = heads & (lane < 63 ? (2ull << lane) - 1ull : ~0ull);
+    WaveRuns r;
+    r.heads = heads;
+    r.len = end - lane;
+    r.head = upto ? 63 - __builtin_clzll(upto) : 0;
+    return r;
+}
+
 template <int D>
 __device__ __forceinline__ void cells_range(const Consts &c, const Roles &R, const Scratch &S,
                                             const int cd[3], int base, int ipt, int n) {
     for (int k = 0; k < ipt; ++k) {
+        if (base + k * kBlock >= n) break;  // uniform per block
         const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= n) break;
-        double p[3] = {0.0, 0.0, 0.0};
+        const bool valid = i < n;
+        int cid = -1;
+        if (valid) {
+            double p[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < D; ++q) {
-            double xv = R.x[q][i];
-            if (!c.wall[q]) {
-                const int mult = xv < c.lbb[q] ? 1 : (xv > c.tfr[q] ? -1 : 0);
-                xv = xv + (c.tfr[q] - c.lbb[q]) * (double)mult;
-                R.x[q][i] = xv;
-                if (mult) R.wc[q][i] -= mult;
+            for (int q = 0; q < D; ++q) {
+                double xv = R.x[q][i];
+                if (!c.wall[q]) {
+                    const int mult = xv < c.lbb[q] ? 1 : (xv > c.tfr[q] ? -1 : 0);
+                    xv = xv + (c.tfr[q] - c.lbb[q]) * (double)mult;
+                    R.x[q][i] = xv;
+                    if (mult) R.wc[q][i] -= mult;
+                }
+                p[q] = xv;
             }
-            p[q] = xv;
-        }
-        int id[3] = {0, 0, 0};
+            int id[3] = {0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < D; ++q) {
-            const double f = floor(cd[q] * (p[q] - c.lbb[q]) / c.interval[q]);
-            id[q] = f < 0.0 ? 0 : (f >= (double)cd[q] ? cd[q] - 1 : (int)f);
+            for (int q = 0; q < D; ++q) {
+                const double f = floor(cd[q] * (p[q] - c.lbb[q]) / c.interval[q]);
+                id[q] = f < 0.0 ? 0 : (f >= (double)cd[q] ? cd[q] - 1 : (int)f);
+            }
+            cid = id[2] * cd[0] * cd[1] + id[1] * cd[0] + id[0];
+            S.cell_of[i] = cid;
         }
-        const int cid = id[2] * cd[0] * cd[1] + id[1] * cd[0] + id[0];
-        S.cell_of[i] = cid;
-        atomicAdd(&S.cell_count[cid], 1);
+        const WaveRuns wr = wave_runs(cid, valid);
+        if (valid && (int)(threadIdx.x & 63) == wr.head) atomicAdd(&S.cell_count[cid], wr.len);
     }
 }
 
@@ -135,10 +169,19 @@ __global__ __launch_bounds__(kBlock) void k_s_bin(Foam *foams, const int2 *bt) {
     const int n = F->d.count, ipt = F->ipt;
     const int base = cx.lb * kBlock * ipt;
     for (int k = 0; k < ipt; ++k) {
+        if (base + k * kBlock >= n) break;  // uniform per block
         const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= n) break;
-        const int slot = atomicAdd(&S.cell_cursor[S.cell_of[i]], 1);
-        S.members[slot] = i;
+        const bool valid = i < n;
+        const int cid = valid ? S.cell_of[i] : -1;
+        // one atomic per run of equal cells; the run's lanes take consecutive
+        // slots (the order inside a cell segment does not matter: the reorder
+        // ranks by old index)
+        const WaveRuns wr = wave_runs(cid, valid);
+        const int lane = (int)(threadIdx.x & 63);
+        int slot = 0;
+        if (valid && lane == wr.head) slot = atomicAdd(&S.cell_cursor[cid], wr.len);
+        slot = __shfl(slot, wr.head) + (lane - wr.head);
+        if (valid) S.members[slot] = i;
     }
 }
 
