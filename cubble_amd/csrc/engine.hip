@@ -524,7 +524,7 @@ int Engine::read_header(int k, Foam &f) {
     return CUBBLE_OK;
 }
 
-int Engine::grow_lists(int k, int need, bool rebuild) {
+int Engine::grow_lists(int k, int need, bool rebuild, bool keep) {
     Foam f;
     int rc = read_header(k, f);
     if (rc) return rc;
@@ -532,6 +532,11 @@ int Engine::grow_lists(int k, int need, bool rebuild) {
     const int ncap = std::max(need + need / 4 + 4, f.nbr_cap * 2);
     void *nb = arena.alloc((size_t)ncap * f.cap * sizeof(int32_t));
     if (!nb) return fail(CUBBLE_E_NOMEM, "no device memory to grow the neighbour lists");
+    // column q of bubble i is nbr[q * cap + i]: the old lists are the first
+    // nbr_cap columns of the grown block
+    if (keep)
+        E_TRY(hipMemcpyAsync(nb, r.nbr_block, (size_t)f.nbr_cap * f.cap * sizeof(int32_t),
+                             hipMemcpyDeviceToDevice, stream));
     arena.release(r.nbr_block);
     r.nbr_block = nb;
     int32_t *p = static_cast<int32_t *>(nb);
@@ -570,6 +575,13 @@ int Engine::settle(std::vector<Dyn> &d) {
                        : 0;
         if ((rc = write_dyn(k, f.d))) return rc;
         grew = true;
+    }
+    // grow ahead of the next search when the last one came close to the
+    // capacity: an overflow halts the foam until the host settles
+    for (int k = 0; k < (int)d.size(); ++k) {
+        const int cap = foams[k].h.nbr_cap;
+        if (d[k].overflow || d[k].max_nn + std::max(4, cap / 8) <= cap) continue;
+        if ((rc = grow_lists(k, d[k].max_nn, false, true))) return rc;
     }
     if (grew) rc = read_dyn_all(d);
     return rc;
@@ -627,7 +639,10 @@ int Engine::run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d,
         if ((rc = read_dyn_all(d))) return rc;
         const int64_t left = steps_left(d);
         if (left <= 0) break;
-        const int64_t launches = std::max<int64_t>(1, (left + groups - 1) / groups);
+        // rounds of at most kRoundGroups attempt groups, so a halted foam
+        // (capacity growth) waits at most that long for the host
+        const int64_t launches = std::min<int64_t>(std::max(1, kRoundGroups / groups),
+                                                   std::max<int64_t>(1, (left + groups - 1) / groups));
         for (int64_t k = 0; k < launches; ++k) E_TRY(hipGraphLaunch(graph_, stream));
         if ((rc = settle(d))) return rc;
     }

@@ -53,6 +53,10 @@ struct Tuning {
 // Smallest neighbour-search tile, in bubbles.
 constexpr int kMinSearchTile = 32;
 
+// Most attempt groups between two host checks of the foams' flags in
+// run_steps (halts, list-capacity growth).
+constexpr int kRoundGroups = 128;
+
 // Device copies of every foam's buffers and host copies of their headers,
 // taken by Engine::snapshot and put back by Engine::restore.
 struct Snapshot {
@@ -144,7 +148,7 @@ class Engine {
     DeviceArena arena;
 
   private:
-    int grow_lists(int k, int need, bool rebuild);
+    int grow_lists(int k, int need, bool rebuild, bool keep = false);
     int pull_headers();
     Tuning tuning_;
     Foam *dfoams_ = nullptr;

@@ -273,6 +273,29 @@ def test_neighbour_capacity_growth(ora):
     sim.close()
 
 
+def test_lists_grow_ahead_of_overflow():
+    """A list capacity just above the longest list: the host grows the lists,
+    keeping their contents, as soon as a search comes within 1/8 of the
+    capacity (here right after the initial search), before a later search can
+    overflow them; the run equals one with ample capacity, bit for bit."""
+    a = ca.Simulator(config(3, 6000, seed=43))
+    st, _ = a.get_state()
+    longest = int(np.max(np.diff(st["nbr_off"])))
+    b = ca.Simulator(config(3, 6000, seed=43, nbr_cap=longest + 1))
+    assert b.scalars()["nbr_cap"] > longest + 1
+    sb0, _ = b.get_state()
+    assert np.array_equal(st["nbr"], sb0["nbr"]) and np.array_equal(st["nbr_off"], sb0["nbr_off"])
+    a.run_steps(60, GAS)
+    b.run_steps(60, GAS)
+    sa, sca = a.get_state()
+    sb, scb = b.get_state()
+    for key in sa:
+        assert np.array_equal(sa[key], sb[key]), key
+    assert sca["steps"] == scb["steps"] and sca["dt"] == scb["dt"]
+    a.close()
+    b.close()
+
+
 def test_json_create_and_cli_run(tmp_path):
     import os
     path = os.path.join(os.path.dirname(__file__), "golden", "input_small_2d.json")
