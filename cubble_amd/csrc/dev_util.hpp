@@ -16,6 +16,14 @@
 
 namespace cubble_amd {
 
+// Streaming store of a result the next kernel reads: marked nontemporal, so
+// it does not stay dirty in this XCD's L2 (the kernel boundary writes dirty
+// lines back anyway; L2s of different XCDs are not coherent).
+template <typename T>
+__device__ __forceinline__ void stream_store(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 // Velocity buffer as its element type: double (FP64 mode) or float (mixed).
 template <typename V>
 __device__ __forceinline__ V *vel(void *p) {

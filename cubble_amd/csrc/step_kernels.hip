@@ -64,7 +64,9 @@ __device__ __forceinline__ void predict_range(const Consts &c, const Roles &R, D
             int32_t qc[3] = {0, 0, 0};
 #pragma unroll
             for (int q = 0; q < D; ++q) qc[q] = fixed_code(p[q], c.lbb[q], c.interval[q], c.wall[q] != 0);
-            q4[i] = Q4{qc[0], qc[1], qc[2], (float)pr.r};
+            typedef int v4i __attribute__((ext_vector_type(4)));
+            const v4i w = {qc[0], qc[1], qc[2], __float_as_int((float)pr.r)};
+            stream_store(reinterpret_cast<v4i *>(q4 + i), w);
         } else {
             p4[i] = pr;
         }
@@ -303,7 +305,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
 #pragma unroll
             for (int q = 0; q < D; ++q) {
                 vp[q] = (double)(V)vp[q];  // the stored value drives the corrector
-                vel<V>(R.vp[q])[i] = (V)vp[q];
+                stream_store(vel<V>(R.vp[q]) + i, (V)vp[q]);
             }
         }
         double e = 0.0, dist = 0.0;
@@ -314,7 +316,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             xo[q] = R.x[q][i];
             const double corr = xo[q] + h * ((double)vel<V>(R.v[q])[i] + vp[q]);
             xc[q] = corr;
-            R.xp[q][i] = corr;
+            stream_store(R.xp[q] + i, corr);
             e = fmax(e, fabs(pred - corr));
             const double t = corr - R.saved[q][i];
             dist += t * t;
@@ -329,11 +331,11 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             vr *= c.kappa * c.avg_area_in * n / dyn.s_a * (area - S.ov[i]);
             vr += (double)vel<V>(R.vp[3])[i];
             vr *= c.k_param / area;
-            vel<V>(R.vp[3])[i] = (V)vr;
+            stream_store(vel<V>(R.vp[3]) + i, (V)vr);
             vr = (double)(V)vr;  // the stored value drives the corrector
             const double pred = r;
             const double corr = R.x[3][i] + h * ((double)vel<V>(R.v[3])[i] + vr);
-            R.xp[3][i] = corr;
+            stream_store(R.xp[3] + i, corr);
             e = fmax(e, fabs(pred - corr));
             dist += corr - R.saved[3][i];
             r = corr;
@@ -347,9 +349,9 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             }
             a.max_r = fmax(a.max_r, r);
         } else {
-            R.xp[3][i] = pp.r;  // radius unchanged without gas exchange
+            stream_store(R.xp[3] + i, pp.r);  // radius unchanged without gas exchange
         }
-        S.del[i] = (uint8_t)dele;
+        stream_store(S.del + i, (uint8_t)dele);
         a.max_exp = fmax(a.max_exp, dist);
         if (PATH) {
             double t = xo[0] - xc[0];
@@ -360,9 +362,9 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
                 t = xo[2] - xc[2];
                 pd += t * t;
             }
-            R.path_new[i] = R.path[i] + sqrt(pd);
+            stream_store(R.path_new + i, R.path[i] + sqrt(pd));
         }
-        R.err[i] = fmax(e, R.err[i]);
+        stream_store(R.err + i, fmax(e, R.err[i]));
         a.max_err = fmax(a.max_err, e);
     }
 }
