@@ -114,6 +114,7 @@ struct Dyn {
     double bubble_steps;
     double s_a, s_ar, s_oa, s_oar, s_v, max_err, max_r, max_exp;
     double vol_fix;      // multiplier of the pending volume fix
+    double search_max_r; // largest radius at the running search (k_s_cells), 0 otherwise
 };
 
 struct Foam {

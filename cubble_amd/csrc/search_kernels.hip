@@ -60,8 +60,9 @@ __device__ __forceinline__ WaveRuns wave_runs(int key, bool valid) {
 }
 
 template <int D>
-__device__ __forceinline__ void cells_range(const Consts &c, const Roles &R, const Scratch &S,
-                                            const int cd[3], int base, int ipt, int n) {
+__device__ __forceinline__ double cells_range(const Consts &c, const Roles &R, const Scratch &S,
+                                              const int cd[3], int base, int ipt, int n) {
+    double rmax = 0.0;
     for (int k = 0; k < ipt; ++k) {
         if (base + k * kBlock >= n) break;  // uniform per block
         const int i = base + k * kBlock + (int)threadIdx.x;
@@ -88,10 +89,12 @@ __device__ __forceinline__ void cells_range(const Consts &c, const Roles &R, con
             }
             cid = id[2] * cd[0] * cd[1] + id[1] * cd[0] + id[0];
             S.cell_of[i] = cid;
+            rmax = fmax(rmax, R.x[3][i]);
         }
         const WaveRuns wr = wave_runs(cid, valid);
         if (valid && (int)(threadIdx.x & 63) == wr.head) atomicAdd(&S.cell_count[cid], wr.len);
     }
+    return rmax;
 }
 
 __global__ __launch_bounds__(kBlock) void k_s_cells(Foam *foams, const int2 *bt) {
@@ -105,8 +108,14 @@ __global__ __launch_bounds__(kBlock) void k_s_cells(Foam *foams, const int2 *bt)
     int cd[3];
     cell_dims(c, F->d.max_r_cell, F->cell_cap, cd);
     const int base = cx.lb * kBlock * ipt;
-    if (3 == c.dim) cells_range<3>(c, R, S, cd, base, ipt, n);
-    else cells_range<2>(c, R, S, cd, base, ipt, n);
+    double rmax = 3 == c.dim ? cells_range<3>(c, R, S, cd, base, ipt, n)
+                             : cells_range<2>(c, R, S, cd, base, ipt, n);
+    // largest radius of the foam, for the search's reach bound: radii are
+    // positive, so their bit patterns order like the values
+    rmax = block_max(rmax);
+    if (0 == threadIdx.x && rmax > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long *>(&F->d.search_max_r),
+                  (unsigned long long)__double_as_longlong(rmax));
     // every block computed the same grid; no block of this kernel reads it back
     if (0 == cx.lb && 0 == threadIdx.x) {
         F->d.cell_dim[0] = cd[0];
@@ -365,14 +374,15 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
                                                     const int *__restrict__ start_,
                                                     int *__restrict__ nbr, int *__restrict__ nn,
                                                     const int cd[3], size_t cap, int nbr_cap,
-                                                    int cell, int &sum_nn, int &max_nn,
-                                                    int &ovf) {
+                                                    int cell, int i0, int i1, int &sum_nn,
+                                                    int &max_nn, int &ovf) {
     const auto x4 = gmem(x4_);
     const auto x4s = cmem(x4_);
     const auto start = cmem(start_);
     const int lane = (int)(threadIdx.x & 63);
     cell = uniform_i(cell);
-    const int i0 = start[cell], i1 = start[cell + 1];
+    i0 = uniform_i(i0);
+    i1 = uniform_i(i1);
     if (i0 >= i1) return;
     const int ccx = cell % cd[0];
     const int ccy = (cell / cd[0]) % cd[1];
@@ -424,6 +434,7 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
 }
 
 constexpr int kMaxRuns = 18;  // 3 z-rows x 3 y-rows x 2 x-runs
+constexpr int kOwnUnit = 32;  // own bubbles per search work unit
 
 // Neighbour lists of one cell when every periodic axis has at least 5 cells
 // (the common case).  The candidate runs of the cell (at most two per row:
@@ -439,19 +450,31 @@ constexpr int kMaxRuns = 18;  // 3 z-rows x 3 y-rows x 2 x-runs
 // fl(d + L) = fl(L - |d|): the values of the general minimum-image
 // difference, without its test.  Same order, same lists, same rounding as
 // search_cell_general.
+//
+// Pruning by reach: an own bubble i cannot hit a candidate in the
+// neighbouring row of cells below it in z (or y) when its distance to that
+// face of its cell is at least r_i + rmax + skin, rmax the largest radius of
+// the foam (all candidates lie beyond the face).  Each lane holds, for one
+// own bubble, the rows of the 3 x 3 (z, y) stencil it can reach; a chunk of
+// candidates is tested only against the own bubbles that reach a row the
+// chunk touches.  The reach is widened by 1e-9 (relative, and of the box)
+// against rounding, so no hit is ever skipped: same lists.  rmax <= 0 (not
+// known) disables the pruning.
 // `cell` must be wave-uniform.
 template <int D>
 __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4_,
                                             const int *__restrict__ start_,
                                             int *__restrict__ nbr, int *__restrict__ nn,
                                             const int cd[3], size_t cap, int nbr_cap, int cell,
-                                            int &sum_nn, int &max_nn, int &ovf) {
+                                            int i0, int i1, double rmax, int &sum_nn,
+                                            int &max_nn, int &ovf) {
     const auto x4 = gmem(x4_);
     const auto x4s = cmem(x4_);
     const auto start = cmem(start_);
     const int lane = (int)(threadIdx.x & 63);
     cell = uniform_i(cell);
-    const int i0 = start[cell], i1 = start[cell + 1];
+    i0 = uniform_i(i0);
+    i1 = uniform_i(i1);
     if (i0 >= i1) return;
     const int ccx = cell % cd[0];
     const int ccy = (cell / cd[0]) % cd[1];
@@ -493,10 +516,58 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     }
     const double skin = c.skin;
     const double L[3] = {c.interval[0], c.interval[1], c.interval[2]};
+    // rows of the (z, y) stencil: row r = a * 3 + b holds runs 2r, 2r + 1;
+    // its offsets from the own cell in z and y (-1, 0, +1)
+    constexpr int kRows = kRuns / 2;
+    auto offset = [](int v, int cc, int n) {
+        int o = v - cc;
+        if (o > 1) o -= n;
+        if (o < -1) o += n;
+        return o;
+    };
+    // own cell faces in y and z; widened reach against rounding
+    const double ey = L[1] / cd[1], ez = L[2] / cd[2];
+    const double ylo = c.lbb[1] + ccy * ey, yhi = ylo + ey;
+    const double zlo = c.lbb[2] + ccz * ez, zhi = zlo + ez;
+    const double slack = 1e-9 * fmax(L[0], fmax(L[1], L[2]));
+    const bool prune = rmax > 0.0;
     for (int cb = i0; cb < i1; cb += 64) {
         const int nb = uniform_i(min(64, i1 - cb));
         int cnt = 0;
+        // rows each own bubble (one per lane) can reach
+        int reach_rows = (1 << kRows) - 1;
+        if (prune) {
+            const int o = cb + (lane < nb ? lane : nb - 1);
+            const double py = x4[o].y, pz = 3 == D ? x4[o].z : 0.0;
+            const double reach = (x4[o].r + rmax + skin) * (1.0 + 1e-9) + slack;
+            const bool ry_lo = py - ylo < reach, ry_hi = yhi - py < reach;
+            const bool rz_lo = 3 == D && pz - zlo < reach, rz_hi = 3 == D && zhi - pz < reach;
+            reach_rows = 0;
+#pragma unroll
+            for (int a = 0; a < (3 == D ? 3 : 1); ++a) {
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const int oz = 3 == D && a < Z.n ? offset(Z.at(a), ccz, cd[2]) : 0;
+                    const int oy = b < Y.n ? offset(Y.at(b), ccy, cd[1]) : 0;
+                    const bool okz = 0 == oz || (oz < 0 ? rz_lo : rz_hi);
+                    const bool oky = 0 == oy || (oy < 0 ? ry_lo : ry_hi);
+                    if (okz && oky) reach_rows |= 1 << (a * 3 + b);
+                }
+            }
+        }
+        const unsigned long long own_all = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
         for (int p0 = 0; p0 < total; p0 += 64) {
+            // rows this chunk of candidates touches (scalar)
+            int rows = 0;
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const int rs = pre[2 * r];
+                const int re = 2 * r + 2 < kRuns ? pre[2 * r + 2] : total;
+                if (rs < re && rs < p0 + 64 && re > p0) rows |= 1 << r;
+            }
+            const unsigned long long own_mask =
+                own_all & __builtin_amdgcn_ballot_w64((reach_rows & rows) != 0);
+            if (!own_mask) continue;
             const int p = p0 + lane;
             // the run of candidate p is the last one starting at or before
             // p (empty runs start where the next one does and are skipped)
@@ -523,7 +594,8 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             // the own-bubble loop in two versions, so that chunks with no
             // boundary crossing carry no image arithmetic
             auto own_loop = [&](auto flip) {
-                for (int ii = 0; ii < nb; ++ii) {
+                for (unsigned long long todo = own_mask; todo; todo &= todo - 1ull) {
+                    const int ii = __builtin_ctzll(todo);
                     const int i = cb + ii;
                     const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
                     const double pz = 3 == D ? x4s[i].z : 0.0;
@@ -582,6 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
     const Scratch S = F->s;
     const int ncell = F->d.num_cells;
     const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
+    const double rmax = F->d.search_max_r;  // 0 when not computed (list rebuild)
     const int b0 = cx.lb * F->stile;
     int base = 0, end = 0;  // cells of this tile (none past the bubble count)
     if (b0 < F->d.count) {
@@ -595,21 +668,34 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
 #pragma unroll
     for (int k = 0; k < 3; ++k)
         if (k < c.dim && !c.wall[k] && cd[k] < 5) fast = false;
-    for (int cell = base + (int)(threadIdx.x >> 6); cell < end; cell += kWaves) {
-        if (3 == c.dim) {
-            if (fast)
-                search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
-                               sum_nn, max_nn, ovf);
-            else
-                search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap,
-                                       cell, sum_nn, max_nn, ovf);
-        } else {
-            if (fast)
-                search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
-                               sum_nn, max_nn, ovf);
-            else
-                search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap,
-                                       cell, sum_nn, max_nn, ovf);
+    // work units: up to kOwnUnit consecutive own bubbles of one cell, dealt
+    // round-robin to the waves, so a crowded cell (a hundred bubbles and
+    // more once the radii spread) is shared by the waves instead of
+    // serialising one of them; each own bubble's list is still built by one
+    // wave, candidates in ascending order
+    const int w = (int)(threadIdx.x >> 6);
+    const auto cstart = cmem(S.cell_start);
+    int uid = 0;
+    for (int cell = base; cell < end; ++cell) {
+        const int s0 = cstart[cell], s1 = cstart[cell + 1];
+        for (int o0 = s0; o0 < s1; o0 += kOwnUnit, ++uid) {
+            if ((uid & (kWaves - 1)) != w) continue;
+            const int o1 = min(o0 + kOwnUnit, s1);
+            if (3 == c.dim) {
+                if (fast)
+                    search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
+                                   o0, o1, rmax, sum_nn, max_nn, ovf);
+                else
+                    search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap,
+                                           F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
+            } else {
+                if (fast)
+                    search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
+                                   o0, o1, rmax, sum_nn, max_nn, ovf);
+                else
+                    search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap,
+                                           F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
+            }
         }
     }
     sum_nn = block_sum_i(sum_nn);
@@ -648,6 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_s_fin(Foam *foams, int swap) {
             F->d.halt = 1;
         }
         F->d.need_search = 0;
+        F->d.search_max_r = 0.0;  // the next search recomputes it
     }
 }
 
