@@ -69,6 +69,8 @@ __device__ __forceinline__ void mark_block(Foam *F, int lb) {
     }
 }
 
+constexpr int kFilter = 8;  // list entries loaded together when filtering a list
+
 template <int D, typename V>
 __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, size_t cap, int base,
                                            int ipt, int nnew, int h, double vm) {
@@ -105,21 +107,39 @@ __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, siz
         if (hole || marked) {
             const int m = out;
             out = 0;
-            for (int q = 0; q < m; ++q) {
-                const int j = S.nbr[(size_t)q * cap + src];
-                if (S.del[j]) continue;
-                const int jj = j >= nnew ? S.moved_to[j - nnew] : j;
-                // insertion keeps the list ascending (in place when src == i:
-                // writes stay at positions <= q)
-                int pos = out;
-                while (pos > 0) {
-                    const int prev = S.nbr[(size_t)(pos - 1) * cap + i];
-                    if (prev <= jj) break;
-                    S.nbr[(size_t)pos * cap + i] = prev;
-                    --pos;
+            int last = -1;  // largest entry written so far
+            // entries and their deletion marks are loaded kFilter at a time
+            // (independent loads); a chunk is read before any of its slots is
+            // written, and writes stay at positions <= the entries consumed,
+            // so the filter also runs in place (src == i)
+            for (int q0 = 0; q0 < m; q0 += kFilter) {
+                int js[kFilter];
+                uint8_t dl[kFilter];
+#pragma unroll
+                for (int u = 0; u < kFilter; ++u)
+                    js[u] = q0 + u < m ? S.nbr[(size_t)(q0 + u) * cap + src] : -1;
+#pragma unroll
+                for (int u = 0; u < kFilter; ++u) dl[u] = js[u] >= 0 ? S.del[js[u]] : 1;
+#pragma unroll
+                for (int u = 0; u < kFilter; ++u) {
+                    if (dl[u]) continue;
+                    const int j = js[u];
+                    const int jj = j >= nnew ? S.moved_to[j - nnew] : j;
+                    if (jj > last) {  // still ascending: append
+                        S.nbr[(size_t)out * cap + i] = jj;
+                        last = jj;
+                    } else {  // a moved bubble's new index: insertion keeps the order
+                        int pos = out;
+                        while (pos > 0) {
+                            const int prev = S.nbr[(size_t)(pos - 1) * cap + i];
+                            if (prev <= jj) break;
+                            S.nbr[(size_t)pos * cap + i] = prev;
+                            --pos;
+                        }
+                        S.nbr[(size_t)pos * cap + i] = jj;
+                    }
+                    ++out;
                 }
-                S.nbr[(size_t)pos * cap + i] = jj;
-                ++out;
             }
             S.nn[i] = out;
             if (marked) S.dirty[i] = 0;
