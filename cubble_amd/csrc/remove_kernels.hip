@@ -29,6 +29,8 @@ namespace cubble_amd {
 
 namespace {
 
+constexpr int kFilter = 8;  // list entries loaded together when walking a list
+
 __device__ __forceinline__ void emit_block(Foam *F, int lb) {
     const Scratch S = F->s;
     const int n = F->d.count, ipt = F->ipt;
@@ -62,14 +64,18 @@ __device__ __forceinline__ void mark_block(Foam *F, int lb) {
         if (t >= items) break;
         const int s = t < nd ? S.del_list[t] : S.plan_src[t - nd];
         const int m = S.nn[s];
-        for (int q = 0; q < m; ++q) {
-            const int j = S.nbr[(size_t)q * cap + s];
-            if (j < nnew) S.dirty[j] = 1;  // same value from every writer
+        // list entries loaded kFilter at a time (independent loads)
+        for (int q0 = 0; q0 < m; q0 += kFilter) {
+            int js[kFilter];
+#pragma unroll
+            for (int u = 0; u < kFilter; ++u)
+                js[u] = q0 + u < m ? S.nbr[(size_t)(q0 + u) * cap + s] : nnew;
+#pragma unroll
+            for (int u = 0; u < kFilter; ++u)
+                if (js[u] < nnew) S.dirty[js[u]] = 1;  // same value from every writer
         }
     }
 }
-
-constexpr int kFilter = 8;  // list entries loaded together when filtering a list
 
 template <int D, typename V>
 __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, size_t cap, int base,

@@ -202,8 +202,17 @@ __device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, 
         if (i >= n) break;
         const int c = S.cell_of[i];
         const int s0 = S.cell_start[c], s1 = S.cell_start[c + 1];
+        // rank among the cell's members by old index; the members are loaded
+        // eight at a time (independent loads: crowded cells hold a hundred
+        // bubbles and more)
         int rank = 0;
-        for (int s = s0; s < s1; ++s) rank += S.members[s] < i ? 1 : 0;
+        for (int s = s0; s < s1; s += 8) {
+            int mb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) mb[u] = s + u < s1 ? S.members[s + u] : i;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rank += mb[u] < i ? 1 : 0;
+        }
         const int np = s0 + rank;
         double xs[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
