@@ -69,12 +69,13 @@ def kernel_bytes(dim, p, mixed=False):
     }
 
 
-def step_bytes(dim, p):
+def step_bytes(dim, p, mixed=False):
     """Compulsory bytes per bubble-step of a whole accepted step: read x,r, v,
     v_old, saved, path, error; write x,r, v_new, path, error; the 2p list
-    entries."""
+    entries.  Velocities are FP32 in mixed precision."""
     c = dim + 1
-    return 8 * (6 * c + 4) + 8 * p
+    vb = 4 if mixed else 8
+    return 8 * 3 * c + vb * 3 * c + 8 * 4 + 8 * p
 
 
 KERNEL_NAMES = {"predict": "k_predict", "pair": "k_pair", "correct": "k_correct"}
@@ -140,7 +141,8 @@ def leg_batch(ca, args, ip):
     live = sum(x["count"] for x in s1)
     pairs = sum(x["num_pairs"] for x in s1)
     rf["step_bytes_per_bubble"] = round(
-        sum(step_bytes(x["dim"], x["num_pairs"] / max(x["count"], 1)) * x["count"] for x in s1) /
+        sum(step_bytes(x["dim"], x["num_pairs"] / max(x["count"], 1), True) * x["count"]
+            for x in s1) /
         max(live, 1), 2)
     b.close()
     log("batch: %d steps in %.3f s -> %.4g bubble-steps/s" % (args.steps, elapsed, work / elapsed))
