@@ -24,9 +24,10 @@ __device__ __forceinline__ void stream_store(T *p, T v) {
     __builtin_nontemporal_store(v, p);
 }
 
-// Velocity buffer as its element type: double (FP64 mode) or float (mixed).
+// A per-bubble buffer stored as double in FP64 mode and as float in mixed
+// precision (velocities, saved positions, error, overlap), as its element type.
 template <typename V>
-__device__ __forceinline__ V *vel(void *p) {
+__device__ __forceinline__ V *field(void *p) {
     return static_cast<V *>(p);
 }
 

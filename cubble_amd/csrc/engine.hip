@@ -39,7 +39,7 @@ size_t carve_foam(Foam &F, char *base) {
     const int D = F.c.dim;
     double *xb[2][4] = {};
     void *vb[4][4] = {};
-    // velocities: FP32 in mixed precision
+    // velocities, saved positions, error and overlap: FP32 in mixed precision
     const bool fvel = CUBBLE_PREC_MIXED == F.precision;
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < 4; ++q)
@@ -55,12 +55,13 @@ size_t carve_foam(Foam &F, char *base) {
         F.r.vp[q] = vb[1][q];
         F.r.vo[q] = vb[2][q];
         F.r.vs[q] = vb[3][q];
-        F.r.saved[q] = (2 == q && 2 == D) ? nullptr : L.take<double>(cap);
+        F.r.saved[q] = (2 == q && 2 == D) ? nullptr
+                       : fvel ? (void *)L.take<float>(cap) : (void *)L.take<double>(cap);
     }
     F.r.path = L.take<double>(cap);
     F.r.path_new = L.take<double>(cap);
-    F.r.err = L.take<double>(cap);
-    F.r.err_alt = L.take<double>(cap);
+    F.r.err = fvel ? (void *)L.take<float>(cap) : (void *)L.take<double>(cap);
+    F.r.err_alt = fvel ? (void *)L.take<float>(cap) : (void *)L.take<double>(cap);
     for (int q = 0; q < 3; ++q) {
         F.r.wc[q] = q < D ? L.take<int32_t>(cap) : nullptr;
         F.r.wc_alt[q] = q < D ? L.take<int32_t>(cap) : nullptr;
@@ -74,7 +75,7 @@ size_t carve_foam(Foam &F, char *base) {
     S.p4 = mixed ? nullptr : L.take<D4>(cap);
     S.q4 = mixed ? L.take<Q4>(cap) : nullptr;
     S.x4 = L.take<D4>(cap);
-    S.ov = L.take<double>(cap);
+    S.ov = fvel ? (void *)L.take<float>(cap) : (void *)L.take<double>(cap);
     S.del = L.take<uint8_t>(cap);
     S.dirty = L.take<uint8_t>(cap);
     S.nn = L.take<int32_t>(cap);

@@ -37,7 +37,7 @@ struct Consts {
 // Role pointers: which buffer currently plays each part of the state.
 // Velocity buffers hold FP64 in FP64 mode and FP32 in mixed precision, where
 // every velocity is an FP32 result of the pair kernel (plus the corrector's
-// wall and gas terms, rounded once); kernels read them through vel<V>().
+// wall and gas terms, rounded once); kernels read them through field<V>().
 struct Roles {
     double *x[4];        // current x, y, z, r
     double *xp[4];       // predicted, then corrected (attempt output)
@@ -45,9 +45,9 @@ struct Roles {
     void *vp[4];         // predicted velocities (attempt output)
     void *vo[4];         // old velocities
     void *vs[4];         // spare (search reorder target)
-    double *saved[4];    // positions at the last search
+    void *saved[4];      // positions at the last search (FP32 in mixed precision)
     double *path, *path_new;
-    double *err, *err_alt;
+    void *err, *err_alt; // per-bubble error (FP32 in mixed precision)
     int32_t *wc[3], *wc_alt[3];
     int32_t *idx, *idx_alt;
 };
@@ -72,7 +72,7 @@ struct Scratch {
     D4 *p4;              // predicted x, y, z, r of the current attempt (FP64 mode; else null)
     Q4 *q4;              // the same in fixed point + FP32 (mixed precision; else null)
     D4 *x4;              // current x, y, z, r in cell order, written by the search
-    double *ov;          // per-bubble overlap area sum
+    void *ov;            // per-bubble overlap area sum (FP32 in mixed precision)
     uint8_t *del;        // deletion mark of the last attempt
     uint8_t *dirty;      // list needs filtering after a removal (all zero otherwise)
     int32_t *nn;         // neighbour count

@@ -75,9 +75,9 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
             }
         });
 #pragma unroll
-        for (int a = 0; a < D; ++a) stream_store(vel<float>(R.vp[a]) + i, acc[a]);
-        stream_store(vel<float>(R.vp[3]) + i, dr);
-        stream_store(S.ov + i, (double)ovl);
+        for (int a = 0; a < D; ++a) stream_store(field<float>(R.vp[a]) + i, acc[a]);
+        stream_store(field<float>(R.vp[3]) + i, dr);
+        stream_store(field<float>(S.ov) + i, ovl);
         soa += (double)ovl;
         soar += (double)ovl * (1.0 / (double)ri_f);
     }

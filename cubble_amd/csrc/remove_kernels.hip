@@ -99,12 +99,12 @@ __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, siz
             for (int q = 0; q < 4; ++q) {
                 if (2 == q && 2 == D) continue;
                 R.x[q][i] = R.x[q][src];
-                vel<V>(R.v[q])[i] = vel<V>(R.v[q])[src];
-                vel<V>(R.vo[q])[i] = vel<V>(R.vo[q])[src];
-                R.saved[q][i] = R.saved[q][src];
+                field<V>(R.v[q])[i] = field<V>(R.v[q])[src];
+                field<V>(R.vo[q])[i] = field<V>(R.vo[q])[src];
+                field<V>(R.saved[q])[i] = field<V>(R.saved[q])[src];
             }
             R.path[i] = R.path[src];
-            R.err[i] = R.err[src];
+            field<V>(R.err)[i] = field<V>(R.err)[src];
 #pragma unroll
             for (int q = 0; q < D; ++q) R.wc[q][i] = R.wc[q][src];
             R.idx[i] = R.idx[src];

@@ -221,13 +221,13 @@ __device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, 
             const double xv = R.x[q][i];
             xs[q] = xv;
             R.xp[q][np] = xv;
-            R.saved[q][np] = xv;
-            vel<V>(R.vp[q])[np] = vel<V>(R.v[q])[i];
-            vel<V>(R.vs[q])[np] = vel<V>(R.vo[q])[i];
+            field<V>(R.saved[q])[np] = (V)xv;
+            field<V>(R.vp[q])[np] = field<V>(R.v[q])[i];
+            field<V>(R.vs[q])[np] = field<V>(R.vo[q])[i];
         }
         S.x4[np] = D4{xs[0], xs[1], xs[2], xs[3]};
         R.path_new[np] = R.path[i];
-        R.err_alt[np] = R.err[i];
+        field<V>(R.err_alt)[np] = field<V>(R.err)[i];
 #pragma unroll
         for (int q = 0; q < D; ++q) R.wc_alt[q][np] = R.wc[q][i];
         R.idx_alt[np] = R.idx[i];
@@ -269,9 +269,9 @@ __device__ void swap_reordered_roles(Roles &W) {
     double *t = W.path;
     W.path = W.path_new;
     W.path_new = t;
-    t = W.err;
+    void *e = W.err;
     W.err = W.err_alt;
-    W.err_alt = t;
+    W.err_alt = e;
     for (int q = 0; q < 3; ++q) {
         int *u = W.wc[q];
         W.wc[q] = W.wc_alt[q];

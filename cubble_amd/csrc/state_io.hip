@@ -86,9 +86,9 @@ int h2d(Engine &e, void *dst, const void *src, size_t bytes) {
     return CUBBLE_OK;
 }
 
-// Velocity buffers are FP32 in mixed precision, FP64 otherwise; host views
-// are FP64 in both.
-int d2h_vel(Engine &e, const Foam &f, double *dst, const void *src, size_t n) {
+// Velocities, saved positions, error and overlap are FP32 in mixed precision,
+// FP64 otherwise; host views are FP64 in both.
+int d2h_field(Engine &e, const Foam &f, double *dst, const void *src, size_t n) {
     if (!dst || !n) return CUBBLE_OK;
     if (CUBBLE_PREC_MIXED != f.precision) return d2h(e, dst, src, n * sizeof(double));
     std::vector<float> t(n);
@@ -98,7 +98,7 @@ int d2h_vel(Engine &e, const Foam &f, double *dst, const void *src, size_t n) {
     return CUBBLE_OK;
 }
 
-int h2d_vel(Engine &e, const Foam &f, void *dst, const double *src, size_t n) {
+int h2d_field(Engine &e, const Foam &f, void *dst, const double *src, size_t n) {
     if (!src || !n) return CUBBLE_OK;
     if (CUBBLE_PREC_MIXED != f.precision) return h2d(e, dst, src, n * sizeof(double));
     std::vector<float> t(n);
@@ -121,12 +121,13 @@ int get_state(Engine &e, int k, cubble_state_view *v) {
                 if (p) std::memset(p, 0, db);
             continue;
         }
-        if ((rc = d2h(e, v->x[q], f.r.x[q], db)) || (rc = d2h_vel(e, f, v->v[q], f.r.v[q], n)) ||
-            (rc = d2h_vel(e, f, v->vo[q], f.r.vo[q], n)) ||
-            (rc = d2h(e, v->saved[q], f.r.saved[q], db)))
+        if ((rc = d2h(e, v->x[q], f.r.x[q], db)) || (rc = d2h_field(e, f, v->v[q], f.r.v[q], n)) ||
+            (rc = d2h_field(e, f, v->vo[q], f.r.vo[q], n)) ||
+            (rc = d2h_field(e, f, v->saved[q], f.r.saved[q], n)))
             return rc;
     }
-    if ((rc = d2h(e, v->path, f.r.path, db)) || (rc = d2h(e, v->error, f.r.err, db))) return rc;
+    if ((rc = d2h(e, v->path, f.r.path, db)) || (rc = d2h_field(e, f, v->error, f.r.err, n)))
+        return rc;
     for (int q = 0; q < 3; ++q) {
         if (q >= f.c.dim) {
             if (v->wrap[q]) std::memset(v->wrap[q], 0, ib);
@@ -173,10 +174,11 @@ int get_attempt(Engine &e, int k, cubble_attempt_view *v) {
             if (v->vp[q]) std::memset(v->vp[q], 0, db);
             continue;
         }
-        if ((rc = d2h(e, v->xc[q], f.r.xp[q], db)) || (rc = d2h_vel(e, f, v->vp[q], f.r.vp[q], n)))
+        if ((rc = d2h(e, v->xc[q], f.r.xp[q], db)) || (rc = d2h_field(e, f, v->vp[q], f.r.vp[q], n)))
             return rc;
     }
-    if ((rc = d2h(e, v->overlap, f.s.ov, db)) || (rc = d2h(e, v->path_new, f.r.path_new, db)) ||
+    if ((rc = d2h_field(e, f, v->overlap, f.s.ov, n)) ||
+        (rc = d2h(e, v->path_new, f.r.path_new, db)) ||
         (rc = d2h(e, v->deleted, f.s.del, n)))
         return rc;
     return CUBBLE_OK;
@@ -193,12 +195,13 @@ int set_state(Engine &e, int k, const cubble_state_view *v, const cubble_scalars
     const size_t db = (size_t)n * sizeof(double), ib = (size_t)n * sizeof(int32_t);
     for (int q = 0; q < 4; ++q) {
         if (2 == q && 2 == f.c.dim) continue;
-        if ((rc = h2d(e, f.r.x[q], v->x[q], db)) || (rc = h2d_vel(e, f, f.r.v[q], v->v[q], n)) ||
-            (rc = h2d_vel(e, f, f.r.vo[q], v->vo[q], n)) ||
-            (rc = h2d(e, f.r.saved[q], v->saved[q], db)))
+        if ((rc = h2d(e, f.r.x[q], v->x[q], db)) || (rc = h2d_field(e, f, f.r.v[q], v->v[q], n)) ||
+            (rc = h2d_field(e, f, f.r.vo[q], v->vo[q], n)) ||
+            (rc = h2d_field(e, f, f.r.saved[q], v->saved[q], n)))
             return rc;
     }
-    if ((rc = h2d(e, f.r.path, v->path, db)) || (rc = h2d(e, f.r.err, v->error, db))) return rc;
+    if ((rc = h2d(e, f.r.path, v->path, db)) || (rc = h2d_field(e, f, f.r.err, v->error, n)))
+        return rc;
     for (int q = 0; q < f.c.dim; ++q)
         if ((rc = h2d(e, f.r.wc[q], v->wrap[q], ib))) return rc;
     if ((rc = h2d(e, f.r.idx, v->index, ib))) return rc;

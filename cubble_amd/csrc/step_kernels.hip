@@ -35,9 +35,9 @@ __device__ __forceinline__ D4 predicted(const Roles &R, double h, int i) {
     double p[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < D; ++q)
-        p[q] = R.x[q][i] + h * (3.0 * (double)vel<V>(R.v[q])[i] - (double)vel<V>(R.vo[q])[i]);
+        p[q] = R.x[q][i] + h * (3.0 * (double)field<V>(R.v[q])[i] - (double)field<V>(R.vo[q])[i]);
     const double rp =
-        GAS ? R.x[3][i] + h * (3.0 * (double)vel<V>(R.v[3])[i] - (double)vel<V>(R.vo[3])[i])
+        GAS ? R.x[3][i] + h * (3.0 * (double)field<V>(R.v[3])[i] - (double)field<V>(R.vo[3])[i])
             : R.x[3][i];
     return D4{p[0], p[1], p[2], rp};
 }
@@ -164,9 +164,9 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
             }
         });
 #pragma unroll
-        for (int a = 0; a < D; ++a) vel<double>(R.vp[a])[i] = acc[a];
-        vel<double>(R.vp[3])[i] = dr;
-        S.ov[i] = ovl;
+        for (int a = 0; a < D; ++a) field<double>(R.vp[a])[i] = acc[a];
+        field<double>(R.vp[3])[i] = dr;
+        field<double>(S.ov)[i] = ovl;
         soa += ovl;
         soar += ovl * inv_ri;
     }
@@ -297,7 +297,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
         const double pv[3] = {pp.x, pp.y, pp.z};
         double vp[3];
 #pragma unroll
-        for (int q = 0; q < D; ++q) vp[q] = (double)vel<V>(R.vp[q])[i];
+        for (int q = 0; q < D; ++q) vp[q] = (double)field<V>(R.vp[q])[i];
         if (walls) {
             double vz = 3 == D ? vp[2] : 0.0;
             add_wall_velocity<D>(c, pp.x, pp.y, pp.z, pp.r, vp[0], vp[1], vz);
@@ -305,7 +305,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
 #pragma unroll
             for (int q = 0; q < D; ++q) {
                 vp[q] = (double)(V)vp[q];  // the stored value drives the corrector
-                stream_store(vel<V>(R.vp[q]) + i, (V)vp[q]);
+                stream_store(field<V>(R.vp[q]) + i, (V)vp[q]);
             }
         }
         double e = 0.0, dist = 0.0;
@@ -314,11 +314,11 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
         for (int q = 0; q < D; ++q) {
             const double pred = pv[q];
             xo[q] = R.x[q][i];
-            const double corr = xo[q] + h * ((double)vel<V>(R.v[q])[i] + vp[q]);
+            const double corr = xo[q] + h * ((double)field<V>(R.v[q])[i] + vp[q]);
             xc[q] = corr;
             stream_store(R.xp[q] + i, corr);
             e = fmax(e, fabs(pred - corr));
-            const double t = corr - R.saved[q][i];
+            const double t = corr - (double)field<V>(R.saved[q])[i];
             dist += t * t;
         }
         dist = sqrt(dist);
@@ -328,16 +328,16 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             double area = 2.0 * kPi * r;
             if (3 == D) area *= 2.0 * r;
             double vr = inv_rho - 1.0 / r;
-            vr *= c.kappa * c.avg_area_in * n / dyn.s_a * (area - S.ov[i]);
-            vr += (double)vel<V>(R.vp[3])[i];
+            vr *= c.kappa * c.avg_area_in * n / dyn.s_a * (area - (double)field<V>(S.ov)[i]);
+            vr += (double)field<V>(R.vp[3])[i];
             vr *= c.k_param / area;
-            stream_store(vel<V>(R.vp[3]) + i, (V)vr);
+            stream_store(field<V>(R.vp[3]) + i, (V)vr);
             vr = (double)(V)vr;  // the stored value drives the corrector
             const double pred = r;
-            const double corr = R.x[3][i] + h * ((double)vel<V>(R.v[3])[i] + vr);
+            const double corr = R.x[3][i] + h * ((double)field<V>(R.v[3])[i] + vr);
             stream_store(R.xp[3] + i, corr);
             e = fmax(e, fabs(pred - corr));
-            dist += corr - R.saved[3][i];
+            dist += corr - (double)field<V>(R.saved[3])[i];
             r = corr;
             if (r > c.min_rad) {
                 double vol = r * r;
@@ -364,7 +364,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             }
             stream_store(R.path_new + i, R.path[i] + sqrt(pd));
         }
-        stream_store(R.err + i, fmax(e, R.err[i]));
+        stream_store(field<V>(R.err) + i, (V)fmax(e, (double)field<V>(R.err)[i]));
         a.max_err = fmax(a.max_err, e);
     }
 }
