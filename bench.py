@@ -52,7 +52,9 @@ def kernel_bytes(dim, p, mixed=False):
     kernel reads only that, the corrector recomputes the prediction)."""
     c = dim + 1
     rec = 16 if mixed else 8 * c
-    vb = 4 if mixed else 8  # bytes per velocity component (FP32 in mixed precision)
+    # bytes per velocity component, saved position, error and overlap value
+    # (FP32 in mixed precision)
+    vb = 4 if mixed else 8
     return {
         # read x, r, v, v_old; write the predicted {x, y, z, r} record (in
         # mixed precision only the 16-byte fixed-point record)
@@ -60,22 +62,23 @@ def kernel_bytes(dim, p, mixed=False):
         # read own record, neighbour count, 2p neighbour indices (the gathered
         # neighbour records are other bubbles' own records: L2 hits when the
         # lists are cell ordered); write v_p incl. the pair dR/dt, overlap
-        "pair": rec + 4 + 8 * p + vb * c + 8,
+        "pair": rec + 4 + 8 * p + vb * c + vb,
         # read predicted record (mixed precision: v_old, to recompute it), x, v,
         # v_p, saved (c each), overlap, path, error; write corrected x, dR/dt,
         # path, error (+1 byte deletion mark)
-        "correct": (vb if mixed else 8) * c + 8 * c + 2 * vb * c + 8 * c + 24
-                   + 8 * c + vb + 17,
+        "correct": (vb if mixed else 8) * c + 8 * c + 2 * vb * c + vb * c + 2 * vb + 8
+                   + 8 * c + vb + vb + 9,
     }
 
 
 def step_bytes(dim, p, mixed=False):
     """Compulsory bytes per bubble-step of a whole accepted step: read x,r, v,
     v_old, saved, path, error; write x,r, v_new, path, error; the 2p list
-    entries.  Velocities are FP32 in mixed precision."""
+    entries.  Velocities, saved positions and error are FP32 in mixed
+    precision."""
     c = dim + 1
     vb = 4 if mixed else 8
-    return 8 * 3 * c + vb * 3 * c + 8 * 4 + 8 * p
+    return 8 * 2 * c + vb * 4 * c + 8 * 2 + vb * 2 + 8 * p
 
 
 KERNEL_NAMES = {"predict": "k_predict", "pair": "k_pair", "correct": "k_correct"}
