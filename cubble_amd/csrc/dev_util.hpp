@@ -277,6 +277,22 @@ __device__ __forceinline__ int fixed_diff(int32_t a, int32_t b) {
     return (int)((uint32_t)a - (uint32_t)b);
 }
 
+// FP64 reciprocal and reciprocal square root: the hardware approximations
+// (v_rcp_f64, v_rsq_f64) refined by one Newton step with FMA, within a few
+// ulp of the correctly rounded 1 / x and 1 / sqrt(x) -- far inside the FP64
+// path's 1e-12 tolerance, at a fraction of the cost of IEEE division and
+// square root.  x > 0 and finite.
+__device__ __forceinline__ double rcp_nr(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    return fma(fma(-hx * y, y, 0.5), y, y);  // y (1.5 - x y^2 / 2)
+}
+
 // Squared length, x, y, z summed in order: reference src/vec.h:41-49.
 __device__ __forceinline__ double sq_len(const double d[3]) {
     double t = 0;

@@ -137,7 +137,8 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
             wrapped_diff<D>(c, xi, yi, zi, pj.x, pj.y, pj.z, d);
             const double m = sq_len(d);
             if (radii * radii >= m) {
-                const double s = 1.0 / sqrt(m) - 1.0 / radii;
+                const double rs = rsq_nr(m);
+                const double s = rs - rcp_nr(radii);
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] += d[a] * f0 * s;
                 if (GAS) {
@@ -149,7 +150,7 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
                     } else {
                         oa = 0.5 * (r2sq - r1sq + m);
                         oa *= oa;
-                        oa /= m;
+                        oa *= rs * rs;  // / m
                         oa = r2sq - oa;
                         oa = oa < 0 ? -oa : oa;
                     }
@@ -159,7 +160,7 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
                         oa = 2.0 * sqrt(oa);
                     }
                     ovl += oa;
-                    dr += oa * (1.0 / rj - inv_ri);
+                    dr += oa * (rcp_nr(rj) - inv_ri);
                 }
             }
         });

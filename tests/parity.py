@@ -70,7 +70,11 @@ def attempt_errors(dim, g, o, f0):
     out["radius"] = scaled_err(g["xc"][3], o["xc"][3], 1.0)
     s_ov = max(float(np.max(np.abs(o["ov"]))), 1e-300)
     out["overlap"] = scaled_err(g["ov"], o["ov"], s_ov)
-    out["path"] = scaled_err(g["path_new"], o["path_new"], 1e-3)
+    # the path grows by |x_c - x|, a difference of coordinates of size up to
+    # the box, so it carries their absolute rounding: it is held to the
+    # positions' bar, relative to the largest coordinate
+    s_x = max(float(np.max(np.abs(o["xc"][k]))) for k in range(dim))
+    out["path"] = scaled_err(g["path_new"], o["path_new"], max(s_x, 1e-3))
     return out
 
 
