@@ -8,8 +8,10 @@
 // (the wrap is the integer wrap-around on periodic axes), rounded once to
 // FP32; a code unit is L / 2^32 (L / 2^30 on wall axes), so quantisation adds
 // at most L / 2^32 per component.  The Durian velocity, overlap area and gas
-// flux are evaluated in FP32 with hardware reciprocal square root /
-// reciprocal, and accumulated per bubble in FP32, and stored as FP32 (the
+// flux are evaluated in FP32 with the hardware's one-instruction reciprocal
+// square root, reciprocal and square root (about 1 ulp; the correctly
+// rounded variants compile to division sequences), and accumulated per
+// bubble in FP32, and stored as FP32 (the
 // velocity buffers of mixed precision).  Positions, radii and the corrector's
 // arithmetic stay FP64.  The overlap area uses the product
 // form
@@ -38,7 +40,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
         if (i >= n) break;
         const Q4 qi = load_rec(gmem(q4) + i);
         const float ri_f = qi.r;
-        const float inv_ri = __frcp_rn(ri_f);
+        const float inv_ri = __builtin_amdgcn_rcpf(ri_f);
         float acc[3] = {0.f, 0.f, 0.f};
         float ovl = 0.f, dr = 0.f;
         const int m_nn = nnA[i];
@@ -52,8 +54,8 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
             const float rad = ri_f + rj_f;
             const float over = rad * rad - m;
             if (over >= 0.f) {
-                const float rs = __frsqrt_rn(m);
-                const float s = rs - __frcp_rn(rad);
+                const float rs = __builtin_amdgcn_rsqf(m);
+                const float s = rs - __builtin_amdgcn_rcpf(rad);
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] += dd[a] * (f0 * s);
                 if (GAS) {
@@ -68,9 +70,9 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
                         a2 = over * (m - dl * dl) * (0.25f * rs * rs);
                         a2 = a2 < 0.f ? -a2 : a2;
                     }
-                    const float oa = 3 == D ? a2 * pi_f : 2.0f * __fsqrt_rn(a2);
+                    const float oa = 3 == D ? a2 * pi_f : 2.0f * __builtin_amdgcn_sqrtf(a2);
                     ovl += oa;
-                    dr += oa * (__frcp_rn(rj_f) - inv_ri);
+                    dr += oa * (__builtin_amdgcn_rcpf(rj_f) - inv_ri);
                 }
             }
         });
