@@ -291,6 +291,7 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
     const double h = 0.5 * dyn.dt;
     const bool walls = c.wall[0] || c.wall[1] || (3 == D && c.wall[2]);
     const double inv_rho = (dyn.s_ar - dyn.s_oar) / (dyn.s_a - dyn.s_oa);
+    const double flux = c.kappa * c.avg_area_in * n / dyn.s_a;  // same for every bubble
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
@@ -328,10 +329,10 @@ __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, c
             double r = pp.r;
             double area = 2.0 * kPi * r;
             if (3 == D) area *= 2.0 * r;
-            double vr = inv_rho - 1.0 / r;
-            vr *= c.kappa * c.avg_area_in * n / dyn.s_a * (area - (double)field<V>(S.ov)[i]);
+            double vr = inv_rho - rcp_nr(r);
+            vr *= flux * (area - (double)field<V>(S.ov)[i]);
             vr += (double)field<V>(R.vp[3])[i];
-            vr *= c.k_param / area;
+            vr *= c.k_param * rcp_nr(area);
             stream_store(field<V>(R.vp[3]) + i, (V)vr);
             vr = (double)(V)vr;  // the stored value drives the corrector
             const double pred = r;
