@@ -345,14 +345,16 @@ __device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i,
                                             int &cnt) {
     const int lane = (int)(threadIdx.x & 63);
     const int base = __builtin_amdgcn_readlane(cnt, ii);
-    if ((m >> lane) & 1ull) {
+    const int upd = base + __popcll(m);  // scalar
+    // the hit lanes become the exec mask directly (no per-lane bit test)
+    if (__builtin_amdgcn_inverse_ballot_w64(m)) {
         const int below = (int)__builtin_amdgcn_mbcnt_hi(
             (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         const int pos = base + below;
         auto nbr = (__attribute__((address_space(1))) int *)(nbr_ + i);
-        if (pos < nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;
+        if (upd <= nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;  // all fit (scalar test)
+        else if (pos < nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;
     }
-    const int upd = base + __popcll(m);  // scalar
     cnt = lane == ii ? upd : cnt;
 }
 
