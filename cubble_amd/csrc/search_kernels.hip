@@ -604,25 +604,40 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             }
             // the own-bubble loop in two versions, so that chunks with no
             // boundary crossing carry no image arithmetic
-            auto own_loop = [&](auto flip) {
-                for (unsigned long long todo = own_mask; todo; todo &= todo - 1ull) {
-                    const int ii = __builtin_ctzll(todo);
-                    const int i = cb + ii;
-                    const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
-                    const double pz = 3 == D ? x4s[i].z : 0.0;
-                    double d[3] = {px - pj.x, py - pj.y, 3 == D ? pz - pj.z : 0.0};
-                    if (decltype(flip)::value) {
-                        // d - (+-L) rounds exactly as -+(L - |d|); d - 0 is d
+            // hit mask of own bubble i against the chunk (one ballot per
+            // comparison: each is the comparison's own lane mask, with no
+            // boolean materialised in a register)
+            auto test = [&](auto flip, int i) {
+                const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
+                const double pz = 3 == D ? x4s[i].z : 0.0;
+                double d[3] = {px - pj.x, py - pj.y, 3 == D ? pz - pj.z : 0.0};
+                if (decltype(flip)::value) {
+                    // d - (+-L) rounds exactly as -+(L - |d|); d - 0 is d
 #pragma unroll
-                        for (int k = 0; k < D; ++k) d[k] = d[k] - sh[k];
+                    for (int k = 0; k < D; ++k) d[k] = d[k] - sh[k];
+                }
+                const double maxd = pr + pj.r + skin;
+                return vmask & __builtin_amdgcn_ballot_w64(j != i) &
+                       __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
+            };
+            // two own bubbles per iteration: half the loop and load overhead on
+            // the scalar unit, two independent FP64 chains in flight
+            auto own_loop = [&](auto flip) {
+                unsigned long long todo = own_mask;
+                while (todo) {
+                    const int ia = __builtin_ctzll(todo);
+                    todo &= todo - 1ull;
+                    if (todo) {
+                        const int ib = __builtin_ctzll(todo);
+                        todo &= todo - 1ull;
+                        const unsigned long long ma = test(flip, cb + ia);
+                        const unsigned long long mb = test(flip, cb + ib);
+                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
+                        if (mb) append_hits(mb, ib, cb + ib, j, nbr, (unsigned)cap, nbr_cap, cnt);
+                    } else {
+                        const unsigned long long ma = test(flip, cb + ia);
+                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
                     }
-                    const double maxd = pr + pj.r + skin;
-                    // one ballot per comparison: each is the comparison's own
-                    // lane mask, with no boolean materialised in a register
-                    const unsigned long long m =
-                        vmask & __builtin_amdgcn_ballot_w64(j != i) &
-                        __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
-                    if (m) append_hits(m, ii, i, j, nbr, (unsigned)cap, nbr_cap, cnt);
                 }
             };
             if (anyflip) own_loop(std::true_type{});
@@ -651,12 +666,12 @@ __device__ __forceinline__ int block_lower_bound(const int *__restrict__ a, int 
     return lo;
 }
 
-// Search grid: tile t of a foam holds the cells whose first bubble lies in
-// [t B, t B + B), B = stile bubbles, found by a lower bound on the cell
-// starts; each cell belongs to exactly one tile.  Wave w takes the tile's
-// cells w, w + 4, ...  Tiles sized in bubbles keep the grid proportional to
-// the bubbles of a foam whatever its cell occupancy (2D cells hold a few
-// bubbles, 3D ones a few tens).
+// Search grid: tile t of a foam searches for the own bubbles [t B, t B + B),
+// B = stile bubbles; the cells holding them are found by a lower bound on
+// the cell starts (a cell cut by a tile boundary is walked by both tiles,
+// each for its own bubbles).  Tiles sized in bubbles keep the grid
+// proportional to the bubbles of a foam whatever its cell occupancy (2D
+// cells hold a few bubbles, crowded 3D ones a hundred and more).
 __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt) {
     const Ctx cx = cell_ctx(foams, bt);
     Foam *F = cx.F;
@@ -666,11 +681,14 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
     const int ncell = F->d.num_cells;
     const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
     const double rmax = F->d.search_max_r;  // 0 when not computed (list rebuild)
+    // tile t: own bubbles [t B, t B + B) (B = stile), whatever cells they lie
+    // in; the cells holding them are [base, end)
     const int b0 = cx.lb * F->stile;
-    int base = 0, end = 0;  // cells of this tile (none past the bubble count)
-    if (b0 < F->d.count) {
-        base = block_lower_bound(S.cell_start, ncell, b0);
-        end = min(block_lower_bound(S.cell_start, ncell, b0 + F->stile), ncell);
+    const int b1 = min(b0 + F->stile, F->d.count);
+    int base = 0, end = 0;
+    if (b0 < b1) {
+        base = block_lower_bound(S.cell_start, ncell, b0 + 1) - 1;  // the cell of bubble b0
+        end = min(block_lower_bound(S.cell_start, ncell, b1), ncell);
     }
     const size_t cap = (size_t)F->cap;
     int sum_nn = 0, max_nn = 0, ovf = 0;
@@ -679,16 +697,16 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
 #pragma unroll
     for (int k = 0; k < 3; ++k)
         if (k < c.dim && !c.wall[k] && cd[k] < 5) fast = false;
-    // work units: up to kOwnUnit consecutive own bubbles of one cell, dealt
-    // round-robin to the waves, so a crowded cell (a hundred bubbles and
-    // more once the radii spread) is shared by the waves instead of
-    // serialising one of them; each own bubble's list is still built by one
-    // wave, candidates in ascending order
+    // work units: up to kOwnUnit consecutive own bubbles of one cell (within
+    // the tile), dealt round-robin to the waves, so a crowded cell (a hundred
+    // bubbles and more once the radii spread) is shared by the waves of
+    // several workgroups instead of serialising one wave; each own bubble's
+    // list is still built by one wave, candidates in ascending order
     const int w = (int)(threadIdx.x >> 6);
     const auto cstart = cmem(S.cell_start);
     int uid = 0;
     for (int cell = base; cell < end; ++cell) {
-        const int s0 = cstart[cell], s1 = cstart[cell + 1];
+        const int s0 = max(cstart[cell], b0), s1 = min(cstart[cell + 1], b1);
         for (int o0 = s0; o0 < s1; o0 += kOwnUnit, ++uid) {
             if ((uid & (kWaves - 1)) != w) continue;
             const int o1 = min(o0 + kOwnUnit, s1);
