@@ -38,8 +38,11 @@ extern "C" {
 /* ---- precision modes ----------------------------------------------------- */
 #define CUBBLE_PREC_FP64 0  /* everything in FP64                               */
 #define CUBBLE_PREC_MIXED 1 /* pair forces and gas fluxes in FP32 on relative
-                               coordinates; positions, radii, error estimate and
-                               volume fix in FP64                                */
+                               coordinates (32-bit fixed-point positions);
+                               velocities, saved positions, per-bubble error
+                               and overlap stored in FP32; positions, radii,
+                               the corrector, error estimate and volume fix
+                               in FP64                                           */
 
 typedef struct cubble_sim cubble_sim;
 
