@@ -98,8 +98,21 @@ class _Scalars(ctypes.Structure):
         ("s_oar", ctypes.c_double), ("s_v", ctypes.c_double), ("max_err", ctypes.c_double),
         ("max_r", ctypes.c_double), ("max_exp", ctypes.c_double), ("n_del", ctypes.c_int32),
         ("accepted", ctypes.c_int32), ("need_search", ctypes.c_int32),
-        ("num_pairs", ctypes.c_int32),
+        ("num_pairs", ctypes.c_int32), ("ts_min", ctypes.c_double), ("ts_max", ctypes.c_double),
+        ("ts_sum", ctypes.c_double),
     ]
+
+
+class _Observables(ctypes.Structure):
+    _fields_ = [("total_energy", ctypes.c_double), ("total_volume", ctypes.c_double),
+                ("box_volume", ctypes.c_double), ("mean_r", ctypes.c_double),
+                ("mean_path", ctypes.c_double), ("mean_v", ctypes.c_double * 4)]
+
+
+class _StopRules(ctypes.Structure):
+    _fields_ = [("min_count", ctypes.c_int32), ("use_time", ctypes.c_int32),
+                ("max_radius", ctypes.c_double), ("time_int", ctypes.c_uint64),
+                ("time_frac", ctypes.c_double)]
 
 
 class _KernelTimes(ctypes.Structure):
@@ -160,7 +173,8 @@ EXPORTS = [
     "cubble_server_create", "cubble_server_submit", "cubble_server_run", "cubble_server_result",
     "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
     "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
-    "cubble_batch_retune",
+    "cubble_batch_retune", "cubble_observe", "cubble_set_stop_rules",
+    "cubble_reset_print_window",
 ]
 
 
@@ -219,6 +233,9 @@ def lib():
         L.cubble_version.restype = ctypes.c_char_p
         L.cubble_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
                                           P(_KernelTimes)]
+        L.cubble_observe.argtypes = [vp, P(_Observables), _dp]
+        L.cubble_set_stop_rules.argtypes = [vp, P(_StopRules)]
+        L.cubble_reset_print_window.argtypes = [vp]
         L.cubble_create_tuned.argtypes = [P(_Config), P(_Tuning), P(vp)]
         L.cubble_batch_create.argtypes = [P(_Config), ctypes.c_int32, P(_Tuning), P(vp)]
         L.cubble_batch_size.argtypes = [vp]
@@ -468,6 +485,31 @@ class Simulator:
                                              int(reps), ctypes.byref(t)))
         return {"predict_ms": t.predict_ms, "pair_ms": t.pair_ms, "correct_ms": t.correct_ms,
                 "reps": t.reps}
+
+    def observe(self, energy=False):
+        """Total energy and volume, box volume and the means of r, path and
+        the velocities, reduced on the device; with energy=True also each
+        bubble's energy (state order)."""
+        o = _Observables()
+        e = np.zeros(self.scalars()["count"]) if energy else None
+        self._call(lib().cubble_observe(self._h, ctypes.byref(o),
+                                        _dptr(e) if e is not None and len(e) else None))
+        out = {name: (list(getattr(o, name)) if name == "mean_v" else getattr(o, name))
+               for name, _ in _Observables._fields_}
+        if energy:
+            out["energy"] = e
+        return out
+
+    def set_stop_rules(self, min_count=0, max_radius=0.0, time=None):
+        """Device-side stop rules checked after every accepted step; time is
+        (integer part, fraction) or None."""
+        r = _StopRules(int(min_count), 0 if time is None else 1, float(max_radius),
+                       0 if time is None else int(time[0]), 0.0 if time is None else float(time[1]))
+        self._call(lib().cubble_set_stop_rules(self._h, ctypes.byref(r)))
+
+    def reset_print_window(self):
+        """Zero the per-bubble error and restart the time-step window."""
+        self._call(lib().cubble_reset_print_window(self._h))
 
     def search_neighbors(self):
         self._call(lib().cubble_search_neighbors(self._h))

@@ -23,7 +23,8 @@ CLI = os.path.join(HERE, "cubble-cli")
 ARCH = os.environ.get("CUBBLE_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["step_kernels.hip", "pair_mixed.hip", "search_kernels.hip", "remove_kernels.hip",
-               "engine.hip", "state_io.hip", "runtime.hip", "server.hip", "autotune.hip"]
+               "engine.hip", "state_io.hip", "runtime.hip", "server.hip", "autotune.hip",
+               "observe_kernels.hip"]
 HOST_SOURCES = ["host_init.cpp", "cli.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", INCLUDE, "-I", CSRC]

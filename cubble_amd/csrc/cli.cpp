@@ -1,16 +1,139 @@
 // cli.cpp -- drop-in command line entry, cubble_run(argc, argv), and the
 // one-line main of cubble-cli.  Mirrors cubble::run (reference
-// src/simulator.cu:1161-1186) and the integration loop of simulate()
-// (src/simulator.cu:958-1157): argument check, device check, integrate until
-// the bubble count reaches numEnd or the largest bubble reaches half the box,
-// printing the reference's integration table once per scaled time unit.
+// src/simulator.cu:1161-1186) and simulate() (src/simulator.cu:958-1157):
+// argument check, device check, the integration loop and its stop rule, the
+// printed table, results.dat and the snapshot CSVs.
+//
+// The reference's host loop checks after every accepted step whether to
+// print, snapshot or stop.  Here the steps run on the device
+// (cubble_run_steps) under stop rules set to the next of those events, so
+// each run ends at the accepted step where the reference acts, and every row
+// and file is taken from the state of that step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <utility>
+#include <vector>
 
 #include "cubble_amd.h"
+#include "host_init.hpp"
+
+namespace {
+
+// Most accepted steps per device run between two looks at the state.
+constexpr int64_t kChunk = 1000;
+
+// A scheduled time, split as the reference compares it against its
+// (integer, fraction) clock (src/simulator.cu:1025-1034).
+struct SplitTime {
+    uint64_t i;
+    double f;
+};
+
+SplitTime split(double t) {
+    const uint64_t i = (uint64_t)t;
+    return {i, t - (double)i};
+}
+
+bool reached(const cubble_scalars &s, SplitTime t) {
+    return s.time_int > t.i || (s.time_int == t.i && s.time_frac >= t.f);
+}
+
+bool earlier(SplitTime a, SplitTime b) { return a.i < b.i || (a.i == b.i && a.f < b.f); }
+
+int fail(cubble_sim *sim) {
+    std::fprintf(stderr, "error: %s\n", cubble_last_error(sim));
+    cubble_destroy(sim);
+    return EXIT_FAILURE;
+}
+
+// Snapshot files <name>.csv.<k> (reference saveSnapshot, src/simulator.cu:387-512).
+class Snapshots {
+  public:
+    explicit Snapshots(std::string name) : name_(std::move(name)) {}
+
+    // Starting positions, by original index (src/simulator.cu:901-936).
+    int start(cubble_sim *sim, const cubble_scalars &sc) {
+        if (read(sim, sc)) return CUBBLE_E_HIP;
+        int32_t top = 0;
+        for (int32_t v : idx_) top = v > top ? v : top;
+        for (auto *p : {&px_, &py_, &pz_}) p->assign((size_t)top + 1, 0.0);
+        for (size_t i = 0; i < idx_.size(); ++i) {
+            px_[idx_[i]] = x_[0][i];
+            py_[idx_[i]] = x_[1][i];
+            pz_[idx_[i]] = x_[2][i];
+        }
+        return CUBBLE_OK;
+    }
+
+    int write(cubble_sim *sim, const cubble_scalars &sc) {
+        if (read(sim, sc)) return CUBBLE_E_HIP;
+        std::stringstream ss;
+        ss << name_ << ".csv." << num_++;
+        std::ofstream file(ss.str().c_str(), std::ios::out);
+        if (!file.is_open()) return CUBBLE_OK;  // the reference skips the file silently
+        const double *L = sc.interval;
+        file << "x,y,z,r,vx,vy,vz,vtot,vr,path,energy,displacement,error,index\n";
+        for (size_t i = 0; i < idx_.size(); ++i) {
+            const int32_t ind = idx_[i];
+            const double xi = x_[0][i], yi = x_[1][i], zi = x_[2][i];
+            const double vx = v_[0][i], vy = v_[1][i], vz = v_[2][i];
+            // displacement since the previous snapshot, one period folded
+            // back per axis (src/simulator.cu:455-468)
+            double dx = std::abs(xi - px_[ind]);
+            dx = dx > 0.5 * L[0] ? dx - L[0] : dx;
+            double dy = std::abs(yi - py_[ind]);
+            dy = dy > 0.5 * L[1] ? dy - L[1] : dy;
+            double dz = std::abs(zi - pz_[ind]);
+            dz = dz > 0.5 * L[2] ? dz - L[2] : dz;
+            file << xi - 0.5 * L[0] << "," << yi - 0.5 * L[1] << "," << zi - 0.5 * L[2] << ","
+                 << x_[3][i] << "," << vx << "," << vy << "," << vz << ","
+                 << std::sqrt(vx * vx + vy * vy + vz * vz) << "," << v_[3][i] << "," << path_[i]
+                 << "," << energy_[i] << "," << std::sqrt(dx * dx + dy * dy + dz * dz) << ","
+                 << err_[i] << "," << ind << "\n";
+            px_[ind] = xi;
+            py_[ind] = yi;
+            pz_[ind] = zi;
+        }
+        return CUBBLE_OK;
+    }
+
+  private:
+    int read(cubble_sim *sim, const cubble_scalars &sc) {
+        const size_t n = (size_t)sc.count;
+        cubble_state_view v = {};
+        for (int q = 0; q < 4; ++q) {
+            x_[q].assign(n, 0.0);
+            v_[q].assign(n, 0.0);
+            v.x[q] = x_[q].data();
+            v.v[q] = v_[q].data();
+        }
+        path_.assign(n, 0.0);
+        err_.assign(n, 0.0);
+        idx_.assign(n, 0);
+        energy_.assign(n, 0.0);
+        v.path = path_.data();
+        v.error = err_.data();
+        v.index = idx_.data();
+        if (cubble_get_state(sim, &v) != CUBBLE_OK) return 1;
+        cubble_observables o;
+        return cubble_observe(sim, &o, energy_.data()) != CUBBLE_OK;
+    }
+
+    std::string name_;
+    int num_ = 0;
+    std::vector<double> px_, py_, pz_;
+    std::vector<double> x_[4], v_[4], path_, err_, energy_;
+    std::vector<int32_t> idx_;
+};
+
+}  // namespace
 
 extern "C" int cubble_run(int argc, char **argv) {
     if (argc != 2) {
@@ -26,59 +149,111 @@ extern "C" int cubble_run(int argc, char **argv) {
         return EXIT_FAILURE;
     }
     cubble_config cfg;
-    if (cubble_config_from_json(argv[1], &cfg) != CUBBLE_OK) {
-        std::fprintf(stderr, "error: %s\n", cubble_last_error(nullptr));
+    if (cubble_config_from_json(argv[1], &cfg) != CUBBLE_OK) return fail(nullptr);
+    double snap_freq = 0.0;
+    std::string snap_name, err;
+    if (cubble_amd::parse_snapshot_json(argv[1], snap_freq, snap_name, err) != CUBBLE_OK) {
+        std::fprintf(stderr, "error: %s\n", err.c_str());
         return EXIT_FAILURE;
     }
     cubble_sim *sim = nullptr;
-    if (cubble_create(&cfg, &sim) != CUBBLE_OK) {
-        std::fprintf(stderr, "error: %s\n", cubble_last_error(nullptr));
-        return EXIT_FAILURE;
-    }
-    cubble_scalars sc;
-    cubble_get_scalars(sim, &sc);
-    const double min_interval =
-        0.5 * (3 == sc.dim ? std::fmin(sc.interval[0], std::fmin(sc.interval[1], sc.interval[2]))
-                           : std::fmin(sc.interval[0], sc.interval[1]));
-    const double time_scale = cfg.k_param / (cfg.r_mean * cfg.r_mean);
-    std::printf("%-5s %-8s %-6s %9s %10s %-6s %-9s %-11s\n", "T", "phi", "R", "#b   ", "#pairs",
-                "#steps", "#searches", "ts");
-    cubble_integrate_params ip = {1, 0, 1};
-    cubble_step_result res;
+    if (cubble_create(&cfg, &sim) != CUBBLE_OK) return fail(nullptr);
     int rc = CUBBLE_OK;
-    int64_t printed = 1, last_steps = 0, last_searches = 0;
-    const double vbox = sc.interval[0] * sc.interval[1] * (3 == sc.dim ? sc.interval[2] : 1.0);
+    cubble_scalars sc;
+    cubble_observables obs;
+    if ((rc = cubble_get_scalars(sim, &sc)) != CUBBLE_OK) return fail(sim);
+    const double *L = sc.interval;
+    const double min_interval =
+        0.5 * (3 == sc.dim ? std::fmin(L[0], std::fmin(L[1], L[2])) : std::fmin(L[0], L[1]));
+    // timeScalingFactor = K / avgRad^2 (src/simulator.cu:557-559)
+    const double time_scale = cfg.k_param / (cfg.r_mean * cfg.r_mean);
+    // energy of the starting state, errors cleared (src/simulator.cu:947-951)
+    if (cubble_reset_print_window(sim) || cubble_observe(sim, &obs, nullptr)) return fail(sim);
+    double e1 = obs.total_energy;
+
+    Snapshots snaps(snap_name);
+    int64_t num_snaps = 0;
+    if (snap_freq > 0.0) {
+        if (snaps.start(sim, sc) || snaps.write(sim, sc)) return fail(sim);
+        ++num_snaps;
+    }
+
+    std::printf("\n===========\nIntegration\n===========\n");
+    std::printf("%-5s %-8s %-6s %-11s %9s %10s %-6s %-9s %-11s %-11s %-11s \n", "T", "phi", "R",
+                "dE", "#b   ", "#pairs", "#steps", "#searches", "min ts", "max ts", "avg ts");
+
+    const cubble_integrate_params ip = {1, 0, 1};
+    cubble_step_result res;
+    int64_t printed = 1;
+    int64_t row_steps0 = sc.steps, row_searches0 = sc.searches;
+    // the reference counts the first row's steps from 0 at its first step
+    // (the counter is incremented after the print check, src/simulator.cu:1134),
+    // so its first row reports one step fewer and averages over that count
+    bool first_row = true;
     for (;;) {
-        rc = cubble_run_steps(sim, &ip, 100, &res);
-        if (rc != CUBBLE_OK) break;
-        const bool more = res.count > cfg.num_end && res.max_radius < min_interval &&
-                          (cfg.max_steps <= 0 || res.steps < cfg.max_steps);
-        if (res.time * time_scale >= (double)printed || !more) {
-            std::printf("%-5lld %-#8.6g %-#6.4g %9d %10d %6lld %-9lld %-9.5e\n", (long long)printed,
-                        0.0 * vbox + cfg.phi, 1.0, res.count, res.num_pairs,
-                        (long long)(res.steps - last_steps),
-                        (long long)(res.searches - last_searches), res.dt);
-            ++printed;
-            last_steps = res.steps;
-            last_searches = res.searches;
+        const SplitTime t_print = split((double)printed / time_scale);
+        SplitTime t_snap = {~0ull, 0.0}, next = t_print;
+        if (snap_freq > 0.0) {
+            t_snap = split((double)num_snaps / (snap_freq * time_scale));
+            if (earlier(t_snap, next)) next = t_snap;
         }
+        const cubble_stop_rules rules = {cfg.num_end, 1, min_interval, next.i, next.f};
+        int64_t chunk = kChunk;
+        if (cfg.max_steps > 0) chunk = std::min<int64_t>(chunk, cfg.max_steps - sc.steps);
+        if ((rc = cubble_set_stop_rules(sim, &rules)) ||
+            (rc = cubble_run_steps(sim, &ip, chunk, &res)) || (rc = cubble_get_scalars(sim, &sc)))
+            break;
+        const bool more = sc.count > cfg.num_end && sc.max_r < min_interval &&
+                          (cfg.max_steps <= 0 || sc.steps < cfg.max_steps);
+        const bool print = reached(sc, t_print);
+        if (print) {
+            if ((rc = cubble_observe(sim, &obs, nullptr))) break;
+            const double e2 = obs.total_energy;
+            double de = std::abs(e2 - e1);
+            if (de > 0.0) de *= 2.0 / (e2 + e1);
+            const double rel_rad = obs.mean_r / cfg.r_mean;
+            const double *v = obs.mean_v;
+            std::ofstream results("results.dat", std::ios_base::app);
+            if (results.is_open())
+                results << printed << " " << rel_rad << " " << sc.count << " " << obs.mean_path
+                        << " " << e2 << " " << de << " " << v[0] << " " << v[1] << " " << v[2]
+                        << " " << std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) << " "
+                        << v[3] << "\n";
+            else
+                std::printf("Couldn't open file stream to append results to!\n");
+            const int64_t nsteps = sc.steps - row_steps0 - (first_row ? 1 : 0);
+            std::printf("%-5d %-#8.6g %-#6.4g %-9.5e %9d %10d %6ld %-9ld %-9.5e %-9.5e %-9.5e \n",
+                        (int)printed, obs.total_volume / obs.box_volume, rel_rad, de, sc.count,
+                        sc.num_pairs, (long)nsteps, (long)(sc.searches - row_searches0),
+                        sc.ts_min, sc.ts_max, sc.ts_sum / (double)nsteps);
+            ++printed;
+            e1 = e2;
+            row_steps0 = sc.steps;
+            row_searches0 = sc.searches;
+            first_row = false;
+        }
+        if (snap_freq > 0.0 && reached(sc, t_snap)) {
+            if ((rc = snaps.write(sim, sc))) break;
+            ++num_snaps;
+        }
+        if (print && (rc = cubble_reset_print_window(sim))) break;
         if (!more) break;
     }
-    if (rc != CUBBLE_OK) {
-        std::fprintf(stderr, "error: %s\n", cubble_last_error(sim));
-        cubble_destroy(sim);
-        return EXIT_FAILURE;
-    }
-    if (res.count <= cfg.num_end)
+    if (rc != CUBBLE_OK) return fail(sim);
+    if (sc.count <= cfg.num_end)
         std::printf("Stopping simulation, since the number of bubbles left in the simulation (%d) "
                     "is less than or equal to the specified minimum (%d)\n",
-                    res.count, cfg.num_end);
-    else if (res.max_radius >= min_interval)
+                    sc.count, cfg.num_end);
+    else if (sc.max_r > min_interval)
         std::printf("Stopping simulation, since the radius of the largest bubble (%g) is greater "
-                    "than half the simulation box\n",
-                    res.max_radius);
+                    "than the simulation box (%g, %g, %g)\n",
+                    sc.max_r, L[0], L[1], L[2]);
+    else if (cfg.max_steps > 0 && sc.steps >= cfg.max_steps)
+        std::printf("Stopping simulation after %lld steps (run.maxSteps)\n", (long long)sc.steps);
     else
-        std::printf("Stopping simulation after %lld steps\n", (long long)res.steps);
+        std::printf("Stopping simulation for an unknown reason...\n");
+    if (snap_freq > 0.0 && snaps.write(sim, sc)) return fail(sim);
+    std::printf("Cleaning up...\n");
     cubble_destroy(sim);
     std::printf("Done\n");
     return EXIT_SUCCESS;

@@ -284,6 +284,8 @@ int Engine::add_generated(const cubble_config &cfg, const Tuning &t, const std::
     d.dt = cfg.dt0;
     d.max_r_cell = io.max_r;
     d.need_search = 1;
+    d.ts_min = kTsMinEmpty;
+    d.ts_max = kTsMaxEmpty;
     r.num_end = cfg.num_end;
     r.max_steps = cfg.max_steps;
     tuning_ = t;
@@ -566,14 +568,10 @@ int Engine::settle(std::vector<Dyn> &d) {
     for (int k = 0; k < (int)d.size(); ++k) {
         if (!d[k].overflow) continue;
         // the step was committed; other foams of the batch are unaffected
-        const int64_t limit = d[k].step_limit;
         if ((rc = grow_lists(k, d[k].max_nn, true))) return rc;
         Foam f;
         if ((rc = read_header(k, f))) return rc;
-        f.d.halt = ((limit > 0 && f.d.steps >= limit) ||
-                    (f.d.count_stop > 0 && f.d.count <= f.d.count_stop))
-                       ? 1
-                       : 0;
+        f.d.halt = (stop_reached(f.d) || (f.d.count_stop > 0 && f.d.count <= f.d.count_stop)) ? 1 : 0;
         if ((rc = write_dyn(k, f.d))) return rc;
         grew = true;
     }

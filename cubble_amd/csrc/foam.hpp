@@ -115,7 +115,27 @@ struct Dyn {
     double s_a, s_ar, s_oa, s_oar, s_v, max_err, max_r, max_exp;
     double vol_fix;      // multiplier of the pending volume fix
     double search_max_r; // largest radius at the running search (k_s_cells), 0 otherwise
+    // stop rules checked after every accepted step (0 = off)
+    double r_stop;       // halt once the step's largest radius >= r_stop
+    int32_t time_stop;   // halt once time_int + time_frac >= stop_int + stop_frac
+    uint64_t stop_int;
+    double stop_frac;
+    // extremes and sum of the accepted steps' dt since the last window reset
+    double ts_min, ts_max, ts_sum;
 };
+
+// Empty time-step window (reference src/simulator.cu:980-982).
+constexpr double kTsMinEmpty = 9999999.9;
+constexpr double kTsMaxEmpty = -1.0;
+
+// Step-limit, radius and time stop rules after an accepted step; the count
+// rule is applied after the removal that changes the count.
+__host__ __device__ inline bool stop_reached(const Dyn &d) {
+    if (d.step_limit > 0 && d.steps >= d.step_limit) return true;
+    if (d.r_stop > 0 && d.max_r >= d.r_stop) return true;
+    return d.time_stop &&
+           (d.time_int > d.stop_int || (d.time_int == d.stop_int && d.time_frac >= d.stop_frac));
+}
 
 struct Foam {
     Consts c;

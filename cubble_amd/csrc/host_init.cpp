@@ -437,6 +437,30 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
     return CUBBLE_OK;
 }
 
+int parse_snapshot_json(const char *path, double &frequency, std::string &filename,
+                        std::string &err) {
+    frequency = 0.0;
+    filename = "snapshot";
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        err = std::string("couldn't open input file ") + (path ? path : "(null)");
+        return CUBBLE_E_IO;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    JParser jp(text);
+    JVal root;
+    if (!jp.val(root) || root.kind != JVal::Obj) {
+        err = "malformed JSON: " + jp.err;
+        return CUBBLE_E_IO;
+    }
+    double v = 0.0;
+    if (jnum(root, {"snapShot", "frequency"}, v)) frequency = v;
+    jstr(root, {"snapShot", "filename"}, filename);
+    return CUBBLE_OK;
+}
+
 namespace {
 
 bool read_cache(const char *path, JVal &root) {

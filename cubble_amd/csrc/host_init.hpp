@@ -27,6 +27,11 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
 // input_parameters.json (reference schema) -> config.
 int parse_config_json(const char *path, cubble_config &cfg, std::string &err);
 
+// snapShot.frequency and snapShot.filename of input_parameters.json
+// (reference src/simulator.cu:568-569); frequency 0 when absent.
+int parse_snapshot_json(const char *path, double &frequency, std::string &filename,
+                        std::string &err);
+
 void config_default(cubble_config &cfg);
 
 // Launch shapes chosen by the autotuner, kept in a JSON file keyed by the

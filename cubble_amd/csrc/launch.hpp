@@ -46,4 +46,10 @@ void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear
 // add > 0: every foam halts after `add` more accepted steps; 0: no limit.
 void launch_set_limits(const Grid &g, long long add);
 
+// One foam's observables (h: a current host copy of its header, dev: its
+// device header): out[0..7] = total energy, total volume, then the sums of
+// r, path, dxdt, dydt, dzdt, drdt; energy[i] (if not null) = bubble i's
+// energy.  Uses the foam's block partials.
+void launch_observe(const Foam &h, Foam *dev, double *out, double *energy, hipStream_t stream);
+
 }  // namespace cubble_amd

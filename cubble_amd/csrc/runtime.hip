@@ -335,6 +335,66 @@ int cubble_set_state(cubble_sim *s, const cubble_state_view *v, const cubble_sca
     return s ? set_state(s->e, 0, v, sc) : CUBBLE_E_INVALID;
 }
 
+int cubble_observe(cubble_sim *s, cubble_observables *out, double *energy) {
+    if (!s || !out) return CUBBLE_E_INVALID;
+    Engine &e = s->e;
+    Foam h;
+    int rc = e.read_header(0, h);
+    if (rc) return rc;
+    const int n = h.d.count;
+    // 8 sums, then the per-bubble energies when asked for
+    const size_t bytes = sizeof(double) * (kPartD + (energy ? (size_t)n : 0));
+    double *buf = static_cast<double *>(e.arena.alloc(bytes));
+    if (!buf) return e.fail(CUBBLE_E_NOMEM, "observe: no device memory");
+    launch_observe(h, e.dfoams(), buf, energy ? buf + kPartD : nullptr, e.stream);
+    double sums[kPartD];
+    hipError_t err = hipGetLastError();
+    if (hipSuccess == err) err = hipMemcpyAsync(sums, buf, sizeof(sums), hipMemcpyDeviceToHost, e.stream);
+    if (hipSuccess == err && energy && n > 0)
+        err = hipMemcpyAsync(energy, buf + kPartD, sizeof(double) * n, hipMemcpyDeviceToHost, e.stream);
+    if (hipSuccess == err) err = hipStreamSynchronize(e.stream);
+    e.arena.release(buf);
+    R_TRY(e, err);
+    out->total_energy = sums[0];
+    out->total_volume = sums[1];
+    out->box_volume = h.c.interval[0] * h.c.interval[1] * (3 == h.c.dim ? h.c.interval[2] : 1.0);
+    // getAvg divides the sum by the count (src/simulator.cu:1049)
+    out->mean_r = n > 0 ? sums[2] / n : 0.0;
+    out->mean_path = n > 0 ? sums[3] / n : 0.0;
+    for (int q = 0; q < 4; ++q) out->mean_v[q] = n > 0 ? sums[4 + q] / n : 0.0;
+    return CUBBLE_OK;
+}
+
+int cubble_set_stop_rules(cubble_sim *s, const cubble_stop_rules *r) {
+    if (!s || !r) return CUBBLE_E_INVALID;
+    if (r->min_count < 0 || r->max_radius < 0 || r->time_frac < 0 || r->time_frac >= 1)
+        return s->e.fail(CUBBLE_E_INVALID, "stop rules: negative limit or time fraction outside [0, 1)");
+    Foam h;
+    int rc = s->e.read_header(0, h);
+    if (rc) return rc;
+    h.d.count_stop = r->min_count;
+    h.d.r_stop = r->max_radius;
+    h.d.time_stop = r->use_time ? 1 : 0;
+    h.d.stop_int = r->time_int;
+    h.d.stop_frac = r->time_frac;
+    return s->e.write_dyn(0, h.d);
+}
+
+int cubble_reset_print_window(cubble_sim *s) {
+    if (!s) return CUBBLE_E_INVALID;
+    Engine &e = s->e;
+    Foam h;
+    int rc = e.read_header(0, h);
+    if (rc) return rc;
+    const size_t elem = CUBBLE_PREC_MIXED == h.precision ? sizeof(float) : sizeof(double);
+    // the error role in the header is current (the search swaps it)
+    R_TRY(e, hipMemsetAsync(h.r.err, 0, elem * (size_t)h.cap, e.stream));
+    h.d.ts_min = kTsMinEmpty;
+    h.d.ts_max = kTsMaxEmpty;
+    h.d.ts_sum = 0.0;
+    return e.write_dyn(0, h.d);
+}
+
 int cubble_time_kernels(cubble_sim *s, const cubble_integrate_params *ip, int32_t reps,
                         cubble_kernel_times *out) {
     return s ? time_kernels(s->e, ip, reps, out) : CUBBLE_E_INVALID;

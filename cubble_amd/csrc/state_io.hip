@@ -58,6 +58,9 @@ int get_scalars(Engine &e, int k, cubble_scalars *o) {
     o->searches = f.d.searches;
     o->removals = f.d.removals;
     o->bubble_steps = f.d.bubble_steps;
+    o->ts_min = f.d.ts_min;
+    o->ts_max = f.d.ts_max;
+    o->ts_sum = f.d.ts_sum;
     o->s_a = f.d.s_a;
     o->s_ar = f.d.s_ar;
     o->s_oa = f.d.s_oa;

@@ -422,7 +422,10 @@ __device__ void control(Foam *F, uint32_t flags, const CorrAcc &a) {
         d.need_remove = a.n_del > 0 ? 1 : 0;
     }
     d.need_search = a.max_exp >= 0.5 * c.skin ? 1 : 0;
-    if (d.step_limit > 0 && d.steps >= d.step_limit) d.halt = 1;
+    d.ts_min = fmin(d.ts_min, ndt);
+    d.ts_max = fmax(d.ts_max, ndt);
+    d.ts_sum += ndt;
+    if (stop_reached(d)) d.halt = 1;
 }
 
 template <int D>

@@ -147,6 +147,11 @@ typedef struct cubble_scalars {
     double bubble_steps;
     double s_a, s_ar, s_oa, s_oar, s_v, max_err, max_r, max_exp;
     int32_t n_del, accepted, need_search, num_pairs;
+    double ts_min, ts_max;    /* extremes of the accepted steps' time step
+                                 since the last cubble_reset_print_window
+                                 (empty window: 9999999.9 and -1.0, the
+                                 reference's, src/simulator.cu:980-982)       */
+    double ts_sum;            /* their sum (src/simulator.cu:1021-1023)       */
 } cubble_scalars;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -196,6 +201,51 @@ int cubble_search_neighbors(cubble_sim *sim);
 /* removeBubbles() (src/simulator.cu:180-190) using the deletion marks of the
  * last attempt. */
 int cubble_remove_bubbles(cubble_sim *sim);
+
+/* ---- observables, stop rules and the print window ------------------------ */
+
+/* What the reference's integration loop reports at a printed row and in a
+ * snapshot (src/simulator.cu:1036-1082, :387-512): totalEnergy
+ * (src/simulator.cu:39-58, kernel potentialEnergy src/kernels.cu:941-956),
+ * totalVolume (src/simulator.cu:362-379), boxVolume (:381-385) and the means
+ * of getAvg (:1036-1050), reduced on the device. */
+typedef struct cubble_observables {
+    double total_energy;      /* sum over bubbles i of the sum over overlapping
+                                 neighbours j of (r_i + r_j - |d_ij|)^2      */
+    double total_volume;      /* sum of pi r^2 (2-D) or 4/3 pi r^3 (3-D)      */
+    double box_volume;
+    double mean_r;
+    double mean_path;
+    double mean_v[4];         /* dxdt, dydt, dzdt, drdt                       */
+} cubble_observables;
+
+/* energy (count doubles, or NULL) receives each bubble's energy in the order
+ * of cubble_get_state: the snapshot's energy column (src/simulator.cu:388-392). */
+int cubble_observe(cubble_sim *sim, cubble_observables *out, double *energy);
+
+/* Stop rules the device loop checks after every accepted step, so that
+ * cubble_run_steps returns at the step where the reference's loop acts:
+ * its stop condition (src/simulator.cu:1016-1018) and its print and snapshot
+ * schedule (:1025-1034, :1110-1122).  A run that meets a rule returns early,
+ * with fewer steps done; a rule already met when a run starts ends it after
+ * its first step.  Rules stay set until replaced (all zero = none). */
+typedef struct cubble_stop_rules {
+    int32_t min_count;        /* halt once count <= min_count after a removal
+                                 (0 = off)                                    */
+    int32_t use_time;         /* 1: halt once time >= time_int + time_frac,
+                                 compared as the reference does (integer part
+                                 first, then the fraction)                    */
+    double max_radius;        /* halt once a step's largest radius >= this
+                                 (0 = off)                                    */
+    uint64_t time_int;
+    double time_frac;
+} cubble_stop_rules;
+int cubble_set_stop_rules(cubble_sim *sim, const cubble_stop_rules *rules);
+
+/* What the reference resets after each printed row (src/simulator.cu:1099-1106,
+ * :1127-1131): the per-bubble error array, and the window of time steps that
+ * cubble_scalars.ts_min / ts_max / ts_sum report. */
+int cubble_reset_print_window(cubble_sim *sim);
 
 /* ---- state access (tests, checkpoints) ----------------------------------- */
 int cubble_get_scalars(cubble_sim *sim, cubble_scalars *out);

@@ -152,6 +152,7 @@ def lib():
         L.ora_brute_pairs.restype = ctypes.c_int64
         L.ora_init_synthetic.argtypes = [ctypes.POINTER(InitParams)]
         L.ora_init_synthetic.restype = vp
+        L.ora_observe.argtypes = [vp, _dp, _dp]
         L.ora_cbrt.argtypes = [ctypes.c_double]
         L.ora_cbrt.restype = ctypes.c_double
         u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -278,6 +279,14 @@ class Foam:
             raise RuntimeError("oracle neighbour list larger than expected")
         st["nbr"] = nbr[: st["nbr_off"][n]].copy()
         return st, sc
+
+    def observe(self):
+        """(sums, energy): sums = total energy, total volume, then the sums of
+        r, path, dxdt, dydt, dzdt, drdt; energy = per-bubble energies."""
+        out = np.zeros(8)
+        e = np.zeros(max(1, self.scalars()["count"]))
+        lib().ora_observe(self.h, _dptr(out), _dptr(e))
+        return out, e[: self.scalars()["count"]]
 
     def set_state(self, st, scalars):
         n = int(scalars["count"])

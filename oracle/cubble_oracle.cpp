@@ -864,6 +864,41 @@ void *ora_init_synthetic(const ora_init_params *p) {
     return f;
 }
 
+/* reference src/simulator.cu:39-58 with src/kernels.cu:941-956 (totalEnergy,
+ * potentialEnergy: for each pair, e = r_i + r_j - |wrappedDifference|, e^2
+ * added to both bubbles when e > 0), src/simulator.cu:362-379 with
+ * src/kernels.cu:1013-1024 (totalVolume, calculateVolumes) and the sums
+ * behind getAvg (src/simulator.cu:1036-1050).  Each bubble sums its full
+ * list in ascending neighbour order (the reference adds with atomics, in
+ * unspecified order); bubbles are summed in index order. */
+void ora_observe(void *p, double out[8], double *energy) {
+    const Foam &f = *static_cast<Foam *>(p);
+    const ora_consts &c = f.c;
+    for (int q = 0; q < 8; ++q) out[q] = 0.0;
+    for (int32_t i = 0; i < f.n; ++i) {
+        const double zi = 3 == c.dim ? f.x[2][i] : 0.0;
+        double e = 0.0;
+        for (int32_t k = f.nbr_off[i]; k < f.nbr_off[i + 1]; ++k) {
+            const int32_t j = f.nbr[k];
+            double d[3];
+            wrapped_diff(c, f.x[0][i], f.x[1][i], zi, f.x[0][j], f.x[1][j],
+                         3 == c.dim ? f.x[2][j] : 0.0, d);
+            const double o = f.x[3][i] + f.x[3][j] - std::sqrt(sq_len(d));
+            if (o > 0) e += o * o;
+        }
+        if (energy) energy[i] = e;
+        const double r = f.x[3][i];
+        double vol = r * r * kPi;
+        if (3 == c.dim) vol *= r * 1.33333333333333333333333333;
+        out[0] += e;
+        out[1] += vol;
+        out[2] += r;
+        out[3] += f.path[i];
+        for (int q = 0; q < 4; ++q)
+            if (2 != q || 3 == c.dim) out[4 + q] += f.v[q][i];
+    }
+}
+
 double ora_cbrt(double a) { return det_cbrt(a); }
 
 void ora_splitmix64(uint64_t seed, int32_t n, uint64_t *out) {

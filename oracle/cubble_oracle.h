@@ -117,6 +117,11 @@ void ora_remove(void *f);
 int64_t ora_brute_pairs(void *f, int32_t *out, int64_t cap);
 /* Synthetic initial foam (positions, radii, constants) plus first search. */
 void *ora_init_synthetic(const ora_init_params *p);
+
+/* Observables of the integration loop's printed rows and snapshots:
+ * out[0..7] = total energy, total volume, then the sums of r, path, dxdt,
+ * dydt, dzdt, drdt; energy (count doubles or NULL) = each bubble's energy. */
+void ora_observe(void *f, double out[8], double *energy);
 /* Deterministic cube root used by the volume fix. */
 double ora_cbrt(double a);
 /* Raw outputs of the project RNG, for known-answer tests: SplitMix64 from

@@ -469,3 +469,51 @@ def test_openmp_matches_serial_closely(ora):
     sb, _ = b.get_state()
     assert np.array_equal(sa["idx"], sb["idx"])
     assert np.allclose(sa["x"], sb["x"], rtol=1e-12, atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# observables of the printed rows and snapshots
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("wall", [0, 1])
+def test_observe_two_bubble_energy_closed_form(ora, wall):
+    # a pair straddling x = 0: 0.4 apart through the periodic boundary, 19.6
+    # apart inside the box when x is a wall; a third bubble far from both
+    L = 20.0
+    pos = [(0.2, 5.0), (L - 0.2, 5.0), (10.0, 12.0)]
+    r = [0.5, 0.6, 0.7]
+    f = foam(ora, pos, r, 2, L=L, wall=(wall, 0, 0))
+    sums, e = f.observe()
+    o = 0.0 if wall else (0.5 + 0.6 - 0.4) ** 2
+    assert e[0] == pytest.approx(o, rel=1e-13, abs=0)
+    assert e[1] == pytest.approx(o, rel=1e-13, abs=0)
+    assert e[2] == 0.0
+    assert sums[0] == pytest.approx(2 * o, rel=1e-13, abs=0)
+    assert sums[1] == pytest.approx(PI * sum(x * x for x in r), rel=1e-15)
+    assert sums[2] == pytest.approx(sum(r), rel=1e-15)
+
+
+@pytest.mark.parametrize("dim,n,phi,seed", [(2, 700, 0.9, 11), (3, 900, 0.9, 12)])
+def test_observe_matches_brute_force_minimum_image(ora, dim, n, phi, seed):
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=phi, seed=seed, radius_dist=dim - 2, r_std=0.15)
+    f.run(25)  # nonzero velocities and paths
+    st, sc = f.get_state()
+    L = np.asarray(f.consts()["interval"][:dim])
+    x = st["x"][:dim].T
+    r = st["x"][3]
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.round(d / L)
+    o = r[:, None] + r[None, :] - np.sqrt((d * d).sum(-1))
+    np.fill_diagonal(o, 0.0)
+    e_bf = np.where(o > 0, o * o, 0.0).sum(1)
+    sums, e = f.observe()
+    assert e_bf.sum() > 0
+    assert np.allclose(e, e_bf, rtol=1e-12, atol=1e-15)
+    vol = PI * r ** 2 * (4.0 / 3.0 * r if dim == 3 else 1.0)
+    assert sums[0] == pytest.approx(e_bf.sum(), rel=1e-12)
+    assert sums[1] == pytest.approx(vol.sum(), rel=1e-13)
+    assert sums[2] == pytest.approx(r.sum(), rel=1e-13)
+    assert sums[3] == pytest.approx(st["path"].sum(), rel=1e-13)
+    assert np.abs(st["path"]).sum() > 0
+    for q in range(4):
+        want = st["v"][q].sum() if (q != 2 or dim == 3) else 0.0
+        assert sums[4 + q] == pytest.approx(want, rel=1e-12, abs=1e-12 * np.abs(st["v"][q]).sum())
