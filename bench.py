@@ -328,7 +328,8 @@ def read_counters(folder, counter):
                 if row.get("Counter_Name") != counter:
                     continue
                 # "ns::kernel(args)" -> "kernel"
-                name = row["Kernel_Name"].split("(")[0].split("::")[-1]
+                # "void ns::k_pair_mixed<false>(...)" -> "k_pair_mixed"
+                name = row["Kernel_Name"].split("(")[0].split("<")[0].split("::")[-1]
                 out.setdefault(name, []).append(float(row["Counter_Value"]))
     return out
 

@@ -64,14 +64,17 @@ class _Config(ctypes.Structure):
         ("k_param", ctypes.c_double), ("error_tolerance", ctypes.c_double),
         ("dt0", ctypes.c_double), ("seed", ctypes.c_uint64), ("precision", ctypes.c_int32),
         ("nbr_cap", ctypes.c_int32), ("cell_cap_factor", ctypes.c_double),
-        ("max_steps", ctypes.c_int64),
+        ("max_steps", ctypes.c_int64), ("flow", ctypes.c_int32),
+        ("flow_vel", ctypes.c_double * 3), ("flow_lbb", ctypes.c_double * 3),
+        ("flow_tfr", ctypes.c_double * 3),
     ]
 
 
 class _StateView(ctypes.Structure):
     _fields_ = [("x", _dp * 4), ("v", _dp * 4), ("vo", _dp * 4), ("saved", _dp * 4),
                 ("path", _dp), ("error", _dp), ("wrap", _ip * 3), ("index", _ip),
-                ("nbr_off", _ip), ("nbr", _ip), ("nbr_cap", ctypes.c_int64)]
+                ("nbr_off", _ip), ("nbr", _ip), ("nbr_cap", ctypes.c_int64),
+                ("degree", _ip)]
 
 
 class _AttemptView(ctypes.Structure):
@@ -99,7 +102,8 @@ class _Scalars(ctypes.Structure):
         ("max_r", ctypes.c_double), ("max_exp", ctypes.c_double), ("n_del", ctypes.c_int32),
         ("accepted", ctypes.c_int32), ("need_search", ctypes.c_int32),
         ("num_pairs", ctypes.c_int32), ("ts_min", ctypes.c_double), ("ts_max", ctypes.c_double),
-        ("ts_sum", ctypes.c_double),
+        ("ts_sum", ctypes.c_double), ("flow_lbb", ctypes.c_double * 3),
+        ("flow_tfr", ctypes.c_double * 3), ("flow_vel", ctypes.c_double * 3),
     ]
 
 
@@ -313,12 +317,18 @@ class Config:
     nbr_cap: int = 0
     cell_cap_factor: float = 2.0
     max_steps: int = 0
+    flow: int = 0                 # flow.impose
+    flow_vel: tuple = (0.01, 0.0, 0.0)
+    flow_lbb: tuple = (0.0, 0.475, 0.0)
+    flow_tfr: tuple = (0.4, 0.525, 0.0)
+
+    _VEC = ("rel_dim", "wall", "flow_vel", "flow_lbb", "flow_tfr")
 
     def _c(self):
         c = _Config()
         for f in dataclasses.fields(self):
             v = getattr(self, f.name)
-            if f.name in ("rel_dim", "wall"):
+            if f.name in self._VEC:
                 arr = getattr(c, f.name)
                 for k in range(3):
                     arr[k] = v[k]
@@ -331,7 +341,7 @@ class Config:
         kw = {}
         for f in dataclasses.fields(cls):
             v = getattr(c, f.name)
-            kw[f.name] = tuple(v) if f.name in ("rel_dim", "wall") else v
+            kw[f.name] = tuple(v) if f.name in cls._VEC else v
         return cls(**kw)
 
     @classmethod
@@ -564,6 +574,8 @@ class Simulator:
         off = i32(st["nbr_off"])
         nbr = i32(st["nbr"] if len(st["nbr"]) else [0])
         v.nbr_off, v.nbr, v.nbr_cap = _iptr(off), _iptr(nbr), len(st["nbr"])
+        if st.get("deg") is not None:
+            v.degree = _iptr(i32(st["deg"]))
         s = _Scalars()
         for name, _ in _Scalars._fields_:
             if name in scalars:
@@ -588,7 +600,7 @@ def _read_state(sc, fill):
     st = {"x": np.zeros((4, n)), "v": np.zeros((4, n)), "vo": np.zeros((4, n)),
           "saved": np.zeros((4, n)), "path": np.zeros(n), "err": np.zeros(n),
           "wc": np.zeros((3, n), np.int32), "idx": np.zeros(n, np.int32),
-          "nbr_off": np.zeros(n + 1, np.int32)}
+          "nbr_off": np.zeros(n + 1, np.int32), "deg": np.zeros(n, np.int32)}
     nbr = np.zeros(max(1, sc["nbr_cap"] * n), np.int32)
     v = _StateView()
     for k in range(4):
@@ -598,6 +610,7 @@ def _read_state(sc, fill):
         v.wrap[k] = _iptr(st["wc"][k])
     v.path, v.error, v.index = _dptr(st["path"]), _dptr(st["err"]), _iptr(st["idx"])
     v.nbr_off, v.nbr, v.nbr_cap = _iptr(st["nbr_off"]), _iptr(nbr), nbr.size
+    v.degree = _iptr(st["deg"])
     fill(ctypes.byref(v))
     st["nbr"] = nbr[: st["nbr_off"][n]].copy()
     return st

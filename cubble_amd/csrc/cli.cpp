@@ -182,7 +182,7 @@ extern "C" int cubble_run(int argc, char **argv) {
     std::printf("%-5s %-8s %-6s %-11s %9s %10s %-6s %-9s %-11s %-11s %-11s \n", "T", "phi", "R",
                 "dE", "#b   ", "#pairs", "#steps", "#searches", "min ts", "max ts", "avg ts");
 
-    const cubble_integrate_params ip = {1, 0, 1};
+    const cubble_integrate_params ip = {1, cfg.flow, 1};
     cubble_step_result res;
     int64_t printed = 1;
     int64_t row_steps0 = sc.steps, row_searches0 = sc.searches;

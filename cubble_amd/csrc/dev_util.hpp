@@ -293,6 +293,48 @@ __device__ __forceinline__ double rsq_nr(double x) {
     return fma(fma(-hx * y, y, 0.5), y, y);  // y (1.5 - x y^2 / 2)
 }
 
+// AB2 predicted position and radius of bubble i: reference
+// src/kernels.cu:48-50.  Shared by k_predict and, in mixed precision (where
+// no FP64 record is stored), by k_correct and the pair kernel's flow test, so
+// all round identically (no contraction, whatever the file's setting).
+template <int D, bool GAS, typename V>
+__device__ __forceinline__ D4 predicted(const Roles &R, double h, int i) {
+#pragma clang fp contract(off)
+    double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < D; ++q)
+        p[q] = R.x[q][i] + h * (3.0 * (double)field<V>(R.v[q])[i] - (double)field<V>(R.vo[q])[i]);
+    const double rp =
+        GAS ? R.x[3][i] + h * (3.0 * (double)field<V>(R.v[3])[i] - (double)field<V>(R.vo[3])[i])
+            : R.x[3][i];
+    return D4{p[0], p[1], p[2], rp};
+}
+
+// Imposed flow (reference addFlowVelocity, src/kernels.cu:497-544).  Inside
+// the flow box -- per axis strictly between the faces, or within the
+// bubble's radius of either face -- the bubble takes the imposed velocity;
+// elsewhere the mean velocity of its neighbours: their velocity sum `fv`
+// times 1 / deg, deg being its list length at the last search.  p is the
+// bubble's predicted position and radius (FP64); the term is added to the
+// pair sum `acc`, before the walls.
+template <int D, typename T>
+__device__ __forceinline__ void add_flow(const Consts &c, const D4 &p, int deg, const T fv[3],
+                                         T acc[3]) {
+#pragma clang fp contract(off)
+    const double r2 = p.r * p.r;
+    const double x[3] = {p.x, p.y, p.z};
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const double lo = c.flow_lbb[a] - x[a], hi = c.flow_tfr[a] - x[a];
+        inside = inside && ((x[a] < c.flow_tfr[a] && x[a] > c.flow_lbb[a]) || lo * lo <= r2 ||
+                            hi * hi <= r2);
+    }
+    const double mult = deg > 0 ? 1.0 / (double)deg : 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] += inside ? (T)c.flow_vel[a] : (T)(mult * (double)fv[a]);
+}
+
 // Squared length, x, y, z summed in order: reference src/vec.h:41-49.
 __device__ __forceinline__ double sq_len(const double d[3]) {
     double t = 0;

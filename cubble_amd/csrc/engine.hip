@@ -79,6 +79,7 @@ size_t carve_foam(Foam &F, char *base) {
     S.del = L.take<uint8_t>(cap);
     S.dirty = L.take<uint8_t>(cap);
     S.nn = L.take<int32_t>(cap);
+    S.deg = L.take<int32_t>(cap);
     S.cell_of = L.take<int32_t>(cap);
     S.members = L.take<int32_t>(cap);
     S.del_list = L.take<int32_t>(cap);

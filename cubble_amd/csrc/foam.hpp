@@ -32,6 +32,8 @@ struct Consts {
     int32_t wall[3];
     double lbb[3], tfr[3], interval[3];
     double wall_drag, skin, min_rad, f0, k_param, kappa, avg_area_in, vol_mul, tol;
+    // imposed flow: the box [flow_lbb, flow_tfr] and the velocity (f0 units)
+    double flow_lbb[3], flow_tfr[3], flow_vel[3];
 };
 
 // Role pointers: which buffer currently plays each part of the state.
@@ -76,6 +78,8 @@ struct Scratch {
     uint8_t *del;        // deletion mark of the last attempt
     uint8_t *dirty;      // list needs filtering after a removal (all zero otherwise)
     int32_t *nn;         // neighbour count
+    int32_t *deg;        // neighbour count at the last search, moved with the
+                         // bubble by a removal but not reduced (imposed flow)
     int32_t *nbr;        // neighbour lists, column-major [nbr_cap][cap]
     int32_t *cell_of;    // cell of each bubble (old order)
     int32_t *cell_count;
@@ -159,6 +163,7 @@ enum Flags : uint32_t {
     kGas = 1u,
     kPath = 2u,
     kCommit = 4u,
+    kFlow = 8u,
 };
 
 }  // namespace cubble_amd

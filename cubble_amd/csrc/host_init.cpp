@@ -262,6 +262,11 @@ void config_default(cubble_config &cfg) {
     cfg.precision = CUBBLE_PREC_FP64;
     cfg.nbr_cap = 0;
     cfg.cell_cap_factor = 2.0;
+    cfg.flow = 0;
+    cfg.flow_vel[0] = 0.01;
+    cfg.flow_lbb[1] = 0.475;
+    cfg.flow_tfr[0] = 0.4;
+    cfg.flow_tfr[1] = 0.525;
 }
 
 void host_cell_dims(const Consts &c, double max_r, long long cap, int cd[3]) {
@@ -339,6 +344,12 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
     c.k_param = cfg.k_param;
     c.kappa = cfg.kappa;
     c.tol = cfg.error_tolerance;
+    // imposed flow: reference src/simulator.cu:561-565, :812-817
+    for (int k = 0; k < 3; ++k) {
+        c.flow_lbb[k] = c.interval[k] * cfg.flow_lbb[k] + c.lbb[k];
+        c.flow_tfr[k] = c.interval[k] * cfg.flow_tfr[k] + c.lbb[k];
+        c.flow_vel[k] = cfg.flow_vel[k] * c.f0;
+    }
     double vbox = c.interval[0] * c.interval[1];
     if (3 == D) vbox *= c.interval[2];
     const double mult = cfg.phi * vbox / kPi;
@@ -401,9 +412,12 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
     if (jnum(root, {"constants", "K", "value"}, v)) cfg.k_param = v;
     if (jnum(root, {"errorTolerance", "value"}, v)) cfg.error_tolerance = v;
     if (jnum(root, {"rngSeed", "value"}, v)) cfg.seed = (uint64_t)v;
-    if (jnum(root, {"flow", "impose"}, v) && v == 1.0) {
-        err = "flow.impose = 1 is not supported by this build";
-        return CUBBLE_E_UNSUPPORTED;
+    if (jnum(root, {"flow", "impose"}, v)) cfg.flow = v == 1.0;
+    const char *ax[3] = {"x", "y", "z"};
+    for (int k = 0; k < 3; ++k) {
+        if (jnum(root, {"flow", "velocity", ax[k]}, v)) cfg.flow_vel[k] = v;
+        if (jnum(root, {"flow", "lbb", ax[k]}, v)) cfg.flow_lbb[k] = v;
+        if (jnum(root, {"flow", "tfr", ax[k]}, v)) cfg.flow_tfr[k] = v;
     }
     cfg.r_lo = 0.5 * cfg.r_mean;
     cfg.r_hi = 1.5 * cfg.r_mean;

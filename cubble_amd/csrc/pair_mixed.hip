@@ -23,10 +23,13 @@
 
 namespace cubble_amd {
 
-template <int D, bool GAS>
+// FLOW: also the imposed-flow term (the neighbour-velocity sum in FP32, the
+// flow-box test on the bubble's FP64 prediction, recomputed here as the
+// corrector does); same neighbour selection as the FP64 kernel.
+template <int D, bool GAS, bool FLOW>
 __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R,
                                                  const Scratch &S, size_t cap, int base, int ipt,
-                                                 int n, double &soa, double &soar) {
+                                                 int n, double h, double &soa, double &soar) {
     const Q4 *__restrict__ q4 = S.q4;
     const int *__restrict__ nnA = S.nn;
     const int *__restrict__ nbr = S.nbr;
@@ -42,6 +45,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
         const float ri_f = qi.r;
         const float inv_ri = __builtin_amdgcn_rcpf(ri_f);
         float acc[3] = {0.f, 0.f, 0.f};
+        float fv[3] = {0.f, 0.f, 0.f};
         float ovl = 0.f, dr = 0.f;
         const int m_nn = nnA[i];
         for_neighbours(nbr, q4, cap, i, m_nn, [&](int j, const Q4 &qj) {
@@ -53,6 +57,10 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
             const float m = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
             const float rad = ri_f + rj_f;
             const float over = rad * rad - m;
+            if (FLOW && (j < i || over >= 0.f)) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) fv[a] += field<float>(R.v[a])[j];
+            }
             if (over >= 0.f) {
                 const float rs = __builtin_amdgcn_rsqf(m);
                 const float s = rs - __builtin_amdgcn_rcpf(rad);
@@ -76,6 +84,7 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
                 }
             }
         });
+        if (FLOW) add_flow<D>(c, predicted<D, GAS, float>(R, h, i), S.deg[i], fv, acc);
 #pragma unroll
         for (int a = 0; a < D; ++a) stream_store(field<float>(R.vp[a]) + i, acc[a]);
         stream_store(field<float>(R.vp[3]) + i, dr);
@@ -85,6 +94,8 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
     }
 }
 
+// One kernel per FLOW value (the flow variant needs more registers).
+template <bool FLOW>
 __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *bt,
                                                        uint32_t flags) {
     const Ctx cx = bubble_ctx(foams, bt);
@@ -98,12 +109,13 @@ __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *
     const int base = cx.lb * kBlock * ipt;
     double soa = 0.0, soar = 0.0;
     const bool gas = (flags & kGas) != 0;
+    const double h = 0.5 * F->d.dt;
     if (3 == c.dim) {
-        if (gas) pair_range_mixed<3, true>(c, R, S, cap, base, ipt, n, soa, soar);
-        else pair_range_mixed<3, false>(c, R, S, cap, base, ipt, n, soa, soar);
+        if (gas) pair_range_mixed<3, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
+        else pair_range_mixed<3, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
     } else {
-        if (gas) pair_range_mixed<2, true>(c, R, S, cap, base, ipt, n, soa, soar);
-        else pair_range_mixed<2, false>(c, R, S, cap, base, ipt, n, soa, soar);
+        if (gas) pair_range_mixed<2, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
+        else pair_range_mixed<2, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
     }
     soa = block_sum(soa);
     soar = block_sum(soar);
@@ -114,8 +126,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *
 }
 
 void launch_pair_mixed(const Grid &g, uint32_t flags) {
-    hipLaunchKernelGGL(k_pair_mixed, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub,
-                       flags);
+    if (flags & kFlow)
+        hipLaunchKernelGGL(k_pair_mixed<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
+                           g.bt_bub, flags);
+    else
+        hipLaunchKernelGGL(k_pair_mixed<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream,
+                           g.foams, g.bt_bub, flags);
 }
 
 }  // namespace cubble_amd

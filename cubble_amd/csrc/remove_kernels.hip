@@ -108,6 +108,7 @@ __device__ __forceinline__ int apply_range(const Roles &R, const Scratch &S, siz
 #pragma unroll
             for (int q = 0; q < D; ++q) R.wc[q][i] = R.wc[q][src];
             R.idx[i] = R.idx[src];
+            S.deg[i] = S.deg[src];  // the reference moves numNeighbors unchanged
         }
         int out = S.nn[src];
         if (hole || marked) {

@@ -47,12 +47,8 @@ uint32_t flags_of(const cubble_integrate_params *ip) {
     uint32_t f = 0;
     if (!ip || ip->use_gas_exchange) f |= kGas;
     if (!ip || ip->increment_path) f |= kPath;
+    if (ip && ip->use_flow) f |= kFlow;
     return f;
-}
-
-int check_ip(Engine &e, const cubble_integrate_params *ip) {
-    if (ip && ip->use_flow) return e.fail(CUBBLE_E_UNSUPPORTED, "imposed flow is not supported");
-    return CUBBLE_OK;
 }
 
 Tuning tuning_of(const cubble_tuning *t) {
@@ -111,8 +107,7 @@ int build(Engine &e, const cubble_config *cfgs, int n, const Tuning &t) {
 
 int time_kernels(Engine &e, const cubble_integrate_params *ip, int reps, cubble_kernel_times *out) {
     if (!out || reps < 1) return e.fail(CUBBLE_E_INVALID, "bad argument");
-    int rc = check_ip(e, ip);
-    if (rc) return rc;
+    int rc = CUBBLE_OK;
     const uint32_t flags = flags_of(ip);
     const Grid g = e.grid();
     std::vector<Dyn> saved;
@@ -198,8 +193,7 @@ int cubble_create_tuned(const cubble_config *cfg, const cubble_tuning *tuning, c
 int cubble_autotune(cubble_sim *sim, const cubble_integrate_params *ip,
                     const cubble_autotune_opts *opts, cubble_autotune_report *report) {
     if (!sim) return CUBBLE_E_INVALID;
-    const int rc = check_ip(sim->e, ip);
-    return rc ? rc : autotune(sim->e, flags_of(ip) | kCommit, opts, report);
+    return autotune(sim->e, flags_of(ip) | kCommit, opts, report);
 }
 
 int cubble_retune(cubble_sim *sim, const cubble_tuning *tuning) {
@@ -271,8 +265,7 @@ int cubble_generate_foam(const cubble_config *cfg, double *x4, cubble_scalars *s
 int cubble_step(cubble_sim *s, const cubble_integrate_params *ip, cubble_step_result *res) {
     if (!s) return CUBBLE_E_INVALID;
     Engine &e = s->e;
-    int rc = check_ip(e, ip);
-    if (rc) return rc;
+    int rc = CUBBLE_OK;
     launch_set_flags(e.grid(), -1, -1, 1);
     e.enqueue_attempt(flags_of(ip));
     R_TRY(e, hipGetLastError());
@@ -284,8 +277,7 @@ int cubble_step(cubble_sim *s, const cubble_integrate_params *ip, cubble_step_re
 
 int cubble_integrate(cubble_sim *s, const cubble_integrate_params *ip, cubble_step_result *res) {
     if (!s) return CUBBLE_E_INVALID;
-    int rc = check_ip(s->e, ip);
-    if (rc) return rc;
+    int rc = CUBBLE_OK;
     std::vector<Dyn> d;
     if ((rc = s->e.integrate(flags_of(ip) | kCommit, d))) return rc;
     fill_result(d[0], res);
@@ -295,8 +287,7 @@ int cubble_integrate(cubble_sim *s, const cubble_integrate_params *ip, cubble_st
 int cubble_run_steps(cubble_sim *s, const cubble_integrate_params *ip, int64_t num_steps,
                      cubble_step_result *res) {
     if (!s || num_steps < 0) return CUBBLE_E_INVALID;
-    int rc = check_ip(s->e, ip);
-    if (rc) return rc;
+    int rc = CUBBLE_OK;
     std::vector<Dyn> d;
     if ((rc = s->e.run_steps(flags_of(ip) | kCommit, num_steps, d))) return rc;
     fill_result(d[0], res);
@@ -425,8 +416,7 @@ int cubble_batch_size(const cubble_batch *b) { return b ? (int)b->e.foams.size()
 int cubble_batch_integrate(cubble_batch *b, const cubble_integrate_params *ip,
                            cubble_step_result *per_foam) {
     if (!b) return CUBBLE_E_INVALID;
-    int rc = check_ip(b->e, ip);
-    if (rc) return rc;
+    int rc = CUBBLE_OK;
     std::vector<Dyn> d;
     if ((rc = b->e.integrate(flags_of(ip) | kCommit, d))) return rc;
     if (per_foam)
@@ -437,8 +427,7 @@ int cubble_batch_integrate(cubble_batch *b, const cubble_integrate_params *ip,
 int cubble_batch_run_steps(cubble_batch *b, const cubble_integrate_params *ip, int64_t num_steps,
                            cubble_step_result *per_foam) {
     if (!b || num_steps < 0) return CUBBLE_E_INVALID;
-    int rc = check_ip(b->e, ip);
-    if (rc) return rc;
+    int rc = CUBBLE_OK;
     std::vector<Dyn> d;
     if ((rc = b->e.run_steps(flags_of(ip) | kCommit, num_steps, d))) return rc;
     if (per_foam)
@@ -462,8 +451,7 @@ int cubble_batch_time_kernels(cubble_batch *b, const cubble_integrate_params *ip
 int cubble_batch_autotune(cubble_batch *b, const cubble_integrate_params *ip,
                           const cubble_autotune_opts *opts, cubble_autotune_report *report) {
     if (!b) return CUBBLE_E_INVALID;
-    const int rc = check_ip(b->e, ip);
-    return rc ? rc : autotune(b->e, flags_of(ip) | kCommit, opts, report);
+    return autotune(b->e, flags_of(ip) | kCommit, opts, report);
 }
 
 int cubble_batch_retune(cubble_batch *b, const cubble_tuning *tuning) {

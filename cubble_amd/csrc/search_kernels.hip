@@ -358,11 +358,15 @@ __device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i,
     cnt = lane == ii ? upd : cnt;
 }
 
+// The list length goes to nn and, as the degree the imposed flow divides by,
+// to deg (which a removal moves but does not reduce).
 __device__ __forceinline__ void finish_group(int cb, int nb, int cnt, int nbr_cap,
-                                             int *__restrict__ nn, int &sum_nn, int &max_nn,
-                                             int &ovf) {
+                                             int *__restrict__ nn, int *__restrict__ deg,
+                                             int &sum_nn, int &max_nn, int &ovf) {
     if ((int)(threadIdx.x & 63) < nb) {
-        nn[cb + (int)(threadIdx.x & 63)] = cnt < nbr_cap ? cnt : nbr_cap;
+        const int v = cnt < nbr_cap ? cnt : nbr_cap;
+        nn[cb + (int)(threadIdx.x & 63)] = v;
+        deg[cb + (int)(threadIdx.x & 63)] = v;
         sum_nn += cnt;
         max_nn = max(max_nn, cnt);
         if (cnt > nbr_cap) ovf = 1;
@@ -384,6 +388,7 @@ template <int D>
 __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *__restrict__ x4_,
                                                     const int *__restrict__ start_,
                                                     int *__restrict__ nbr, int *__restrict__ nn,
+                                                    int *__restrict__ deg,
                                                     const int cd[3], size_t cap, int nbr_cap,
                                                     int cell, int i0, int i1, int &sum_nn,
                                                     int &max_nn, int &ovf) {
@@ -440,7 +445,7 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
                 }
             }
         }
-        finish_group(cb, nb, cnt, nbr_cap, nn, sum_nn, max_nn, ovf);
+        finish_group(cb, nb, cnt, nbr_cap, nn, deg, sum_nn, max_nn, ovf);
     }
 }
 
@@ -476,6 +481,7 @@ template <int D>
 __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4_,
                                             const int *__restrict__ start_,
                                             int *__restrict__ nbr, int *__restrict__ nn,
+                                            int *__restrict__ deg,
                                             const int cd[3], size_t cap, int nbr_cap, int cell,
                                             int i0, int i1, double rmax, int &sum_nn,
                                             int &max_nn, int &ovf) {
@@ -643,7 +649,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             if (anyflip) own_loop(std::true_type{});
             else own_loop(std::false_type{});
         }
-        finish_group(cb, nb, cnt, nbr_cap, nn, sum_nn, max_nn, ovf);
+        finish_group(cb, nb, cnt, nbr_cap, nn, deg, sum_nn, max_nn, ovf);
     }
 }
 
@@ -712,17 +718,17 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
             const int o1 = min(o0 + kOwnUnit, s1);
             if (3 == c.dim) {
                 if (fast)
-                    search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
-                                   o0, o1, rmax, sum_nn, max_nn, ovf);
+                    search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
+                                   F->nbr_cap, cell, o0, o1, rmax, sum_nn, max_nn, ovf);
                 else
-                    search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap,
+                    search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
                                            F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
             } else {
                 if (fast)
-                    search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap, F->nbr_cap, cell,
-                                   o0, o1, rmax, sum_nn, max_nn, ovf);
+                    search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
+                                   F->nbr_cap, cell, o0, o1, rmax, sum_nn, max_nn, ovf);
                 else
-                    search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, cd, cap,
+                    search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
                                            F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
             }
         }

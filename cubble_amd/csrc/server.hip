@@ -32,7 +32,7 @@ namespace {
 struct KeptState {
     cubble_scalars sc{};
     std::vector<double> x, v, vo, saved, path, err;
-    std::vector<int32_t> wc, idx, nbr_off, nbr;
+    std::vector<int32_t> wc, idx, nbr_off, nbr, deg;
 };
 
 struct Request {
@@ -76,6 +76,7 @@ int keep_state(cubble_server *s, int k, Request &q) {
     ks.err.assign(n, 0.0);
     ks.wc.assign(3 * n, 0);
     ks.idx.assign(n, 0);
+    ks.deg.assign(n, 0);
     ks.nbr_off.assign(n + 1, 0);
     ks.nbr.assign((size_t)ks.sc.nbr_cap * n + 1, 0);
     cubble_state_view v{};
@@ -89,6 +90,7 @@ int keep_state(cubble_server *s, int k, Request &q) {
     v.path = ks.path.data();
     v.error = ks.err.data();
     v.index = ks.idx.data();
+    v.degree = ks.deg.data();
     v.nbr_off = ks.nbr_off.data();
     v.nbr = ks.nbr.data();
     v.nbr_cap = (int64_t)ks.nbr.size();
@@ -323,6 +325,7 @@ int cubble_server_get_state(cubble_server *s, int64_t id, cubble_state_view *v,
     cp(v->path, ks.path.data(), n * sizeof(double));
     cp(v->error, ks.err.data(), n * sizeof(double));
     cp(v->index, ks.idx.data(), n * sizeof(int32_t));
+    cp(v->degree, ks.deg.data(), n * sizeof(int32_t));
     cp(v->nbr_off, ks.nbr_off.data(), (n + 1) * sizeof(int32_t));
     if (v->nbr) {
         if ((int64_t)ks.nbr.size() > v->nbr_cap)

@@ -61,6 +61,11 @@ int get_scalars(Engine &e, int k, cubble_scalars *o) {
     o->ts_min = f.d.ts_min;
     o->ts_max = f.d.ts_max;
     o->ts_sum = f.d.ts_sum;
+    for (int q = 0; q < 3; ++q) {
+        o->flow_lbb[q] = f.c.flow_lbb[q];
+        o->flow_tfr[q] = f.c.flow_tfr[q];
+        o->flow_vel[q] = f.c.flow_vel[q];
+    }
     o->s_a = f.d.s_a;
     o->s_ar = f.d.s_ar;
     o->s_oa = f.d.s_oa;
@@ -138,7 +143,7 @@ int get_state(Engine &e, int k, cubble_state_view *v) {
         }
         if ((rc = d2h(e, v->wrap[q], f.r.wc[q], ib))) return rc;
     }
-    if ((rc = d2h(e, v->index, f.r.idx, ib))) return rc;
+    if ((rc = d2h(e, v->index, f.r.idx, ib)) || (rc = d2h(e, v->degree, f.s.deg, ib))) return rc;
     if (v->nbr_off || v->nbr) {
         std::vector<int32_t> nn(n);
         if ((rc = d2h(e, nn.data(), f.s.nn, ib))) return rc;
@@ -207,7 +212,7 @@ int set_state(Engine &e, int k, const cubble_state_view *v, const cubble_scalars
         return rc;
     for (int q = 0; q < f.c.dim; ++q)
         if ((rc = h2d(e, f.r.wc[q], v->wrap[q], ib))) return rc;
-    if ((rc = h2d(e, f.r.idx, v->index, ib))) return rc;
+    if ((rc = h2d(e, f.r.idx, v->index, ib)) || (rc = h2d(e, f.s.deg, v->degree, ib))) return rc;
     if (v->nbr_off) {
         int maxn = 0;
         for (int i = 0; i < n; ++i) maxn = std::max(maxn, v->nbr_off[i + 1] - v->nbr_off[i]);
@@ -221,6 +226,7 @@ int set_state(Engine &e, int k, const cubble_state_view *v, const cubble_scalars
             for (int q = 0; q < nn[i]; ++q) ell[(size_t)q * n + i] = v->nbr[v->nbr_off[i] + q];
         }
         if ((rc = h2d(e, f.s.nn, nn.data(), ib))) return rc;
+        if (!v->degree && (rc = h2d(e, f.s.deg, nn.data(), ib))) return rc;
         if (maxn > 0)
             R_TRY(e, hipMemcpy2D(f.s.nbr, (size_t)f.cap * sizeof(int32_t), ell.data(),
                                  (size_t)n * sizeof(int32_t), (size_t)n * sizeof(int32_t), maxn,

@@ -27,21 +27,6 @@
 namespace cubble_amd {
 
 // ---------------------------------------------------------------- predictor
-// AB2 predicted position and radius of bubble i: reference
-// src/kernels.cu:48-50.  Shared by k_predict and, in mixed precision (where
-// no FP64 record is stored), by k_correct, so both round identically.
-template <int D, bool GAS, typename V>
-__device__ __forceinline__ D4 predicted(const Roles &R, double h, int i) {
-    double p[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < D; ++q)
-        p[q] = R.x[q][i] + h * (3.0 * (double)field<V>(R.v[q])[i] - (double)field<V>(R.vo[q])[i]);
-    const double rp =
-        GAS ? R.x[3][i] + h * (3.0 * (double)field<V>(R.v[3])[i] - (double)field<V>(R.vo[3])[i])
-            : R.x[3][i];
-    return D4{p[0], p[1], p[2], rp};
-}
-
 // FP64: writes the 32-byte record the FP64 pair kernel gathers.  Mixed
 // (FIXED, FP32 velocities): writes only the 16-byte fixed-point record.
 template <int D, bool GAS, bool FIXED>
@@ -111,7 +96,12 @@ __global__ __launch_bounds__(kBlock) void k_predict(Foam *foams, const int2 *bt,
 }
 
 // ---------------------------------------------------------------- pair (FP64)
-template <int D, bool GAS>
+// FLOW: also the imposed-flow term.  The neighbour-velocity sum takes every
+// listed neighbour with a smaller index and the overlapping ones with a
+// larger index: the reference adds dxdt[idx1] into idx2 for every listed pair
+// (idx1 < idx2) but dxdt[idx2] into idx1 only past its overlap test
+// (src/kernels.cu:191-208 vs :281-307).
+template <int D, bool GAS, bool FLOW>
 __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, const Scratch &S,
                                            size_t cap, int base, int ipt, int n, double &soa,
                                            double &soar) {
@@ -126,6 +116,7 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
         const double xi = pi.x, yi = pi.y, zi = pi.z, ri = pi.r;
         const double inv_ri = 1.0 / ri;
         double acc[3] = {0.0, 0.0, 0.0};
+        double fv[3] = {0.0, 0.0, 0.0};
         double ovl = 0.0, dr = 0.0;
         const int m_nn = nnA[i];
         for_neighbours(nbr, p4, cap, i, m_nn, [&](int j, const D4 &pj) {
@@ -136,6 +127,10 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
             double d[3];
             wrapped_diff<D>(c, xi, yi, zi, pj.x, pj.y, pj.z, d);
             const double m = sq_len(d);
+            if (FLOW && (j < i || radii * radii >= m)) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) fv[a] += field<double>(R.v[a])[j];
+            }
             if (radii * radii >= m) {
                 const double rs = rsq_nr(m);
                 const double s = rs - rcp_nr(radii);
@@ -164,6 +159,7 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
                 }
             }
         });
+        if (FLOW) add_flow<D>(c, pi, S.deg[i], fv, acc);
 #pragma unroll
         for (int a = 0; a < D; ++a) field<double>(R.vp[a])[i] = acc[a];
         field<double>(R.vp[3])[i] = dr;
@@ -186,6 +182,9 @@ __device__ __forceinline__ double reduce_parts_max(const double *part, int nb, i
     return block_max(a);
 }
 
+// One kernel per FLOW value: the flow variant needs more registers, which
+// would otherwise lower the occupancy of the plain one.
+template <bool FLOW>
 __global__ __launch_bounds__(kBlock) void k_pair(Foam *foams, const int2 *bt, uint32_t flags) {
     const Ctx cx = bubble_ctx(foams, bt);
     Foam *F = cx.F;
@@ -199,11 +198,11 @@ __global__ __launch_bounds__(kBlock) void k_pair(Foam *foams, const int2 *bt, ui
     double soa = 0.0, soar = 0.0;
     const bool gas = (flags & kGas) != 0;
     if (3 == c.dim) {
-        if (gas) pair_range<3, true>(c, R, S, cap, base, ipt, n, soa, soar);
-        else pair_range<3, false>(c, R, S, cap, base, ipt, n, soa, soar);
+        if (gas) pair_range<3, true, FLOW>(c, R, S, cap, base, ipt, n, soa, soar);
+        else pair_range<3, false, FLOW>(c, R, S, cap, base, ipt, n, soa, soar);
     } else {
-        if (gas) pair_range<2, true>(c, R, S, cap, base, ipt, n, soa, soar);
-        else pair_range<2, false>(c, R, S, cap, base, ipt, n, soa, soar);
+        if (gas) pair_range<2, true, FLOW>(c, R, S, cap, base, ipt, n, soa, soar);
+        else pair_range<2, false, FLOW>(c, R, S, cap, base, ipt, n, soa, soar);
     }
     soa = block_sum(soa);
     soar = block_sum(soar);
@@ -499,7 +498,12 @@ void launch_predict(const Grid &g, uint32_t flags) {
 }
 
 void launch_pair(const Grid &g, uint32_t flags) {
-    hipLaunchKernelGGL(k_pair, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub, flags);
+    if (flags & kFlow)
+        hipLaunchKernelGGL(k_pair<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
+                           g.bt_bub, flags);
+    else
+        hipLaunchKernelGGL(k_pair<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
+                           g.bt_bub, flags);
 }
 
 void launch_sums(const Grid &g) {

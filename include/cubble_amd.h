@@ -47,8 +47,8 @@ extern "C" {
 typedef struct cubble_sim cubble_sim;
 
 /* Mirrors the reference's IntegrationParams flags
- * (src/data_definitions.h:227-236). use_flow must be 0: imposed flow is not
- * part of this build. */
+ * (src/data_definitions.h:227-236).  use_flow adds the imposed flow of the
+ * handle's configuration (flow.*; addFlowVelocity, src/kernels.cu:497-544). */
 typedef struct cubble_integrate_params {
     int32_t use_gas_exchange;
     int32_t use_flow;
@@ -99,6 +99,10 @@ typedef struct cubble_config {
     int32_t nbr_cap;          /* initial neighbour capacity per bubble (0 = default) */
     double cell_cap_factor;   /* cell capacity / initial cells (0 = default 2) */
     int64_t max_steps;        /* CLI stop rule: accepted steps (0 = no limit) */
+    int32_t flow;             /* flow.impose: the CLI integrates with use_flow */
+    double flow_vel[3];       /* flow.velocity, in units of f0 = sigma R / mu  */
+    double flow_lbb[3];       /* flow.lbb, relative to the box (0..1)          */
+    double flow_tfr[3];       /* flow.tfr, relative to the box (0..1)          */
 } cubble_config;
 
 /* State view: caller-owned host buffers, filled by cubble_get_state or read
@@ -116,6 +120,11 @@ typedef struct cubble_state_view {
     int32_t *nbr_off;         /* count + 1 offsets                           */
     int32_t *nbr;             /* neighbour list                              */
     int64_t nbr_cap;          /* length of nbr (get: capacity, set: length)  */
+    int32_t *degree;          /* list length at the last search, which the
+                                 imposed flow divides by (the reference's
+                                 numNeighbors: a removal moves it with the
+                                 bubble but does not reduce it); set: NULL =
+                                 the current list lengths                      */
 } cubble_state_view;
 
 /* Outputs of the last step() attempt (the reference's predicted buffers). */
@@ -152,6 +161,9 @@ typedef struct cubble_scalars {
                                  (empty window: 9999999.9 and -1.0, the
                                  reference's, src/simulator.cu:980-982)       */
     double ts_sum;            /* their sum (src/simulator.cu:1021-1023)       */
+    double flow_lbb[3], flow_tfr[3]; /* flow box, box coordinates
+                                        (src/simulator.cu:812-817)             */
+    double flow_vel[3];       /* imposed velocity times f0 (:561-565)          */
 } cubble_scalars;
 
 /* ---- lifetime ------------------------------------------------------------ */

@@ -8,7 +8,8 @@ State dictionaries use the same keys as ``cubble_amd``'s state views:
 ``x`` (4, n) holds x, y, z, r; ``v``/``vo``/``saved`` (4, n) the matching
 velocities, old velocities and positions at the last search; ``path``,
 ``err`` (n,); ``wc`` (3, n) and ``idx`` (n,) int32; ``nbr_off`` (n+1,) and
-``nbr`` the CSR full neighbour lists.
+``nbr`` the CSR full neighbour lists; ``deg`` (n,) the list lengths at the
+last search (the divisor of the imposed flow's neighbour mean).
 """
 import ctypes
 import os
@@ -40,6 +41,9 @@ class Consts(ctypes.Structure):
         ("vol_mul", ctypes.c_double),
         ("tol", ctypes.c_double),
         ("cell_cap", ctypes.c_int32),
+        ("flow_lbb", ctypes.c_double * 3),
+        ("flow_tfr", ctypes.c_double * 3),
+        ("flow_vel", ctypes.c_double * 3),
     ]
 
 
@@ -84,6 +88,7 @@ class State(ctypes.Structure):
         ("nbr_off", _ip),
         ("nbr", _ip),
         ("nbr_cap", ctypes.c_int64),
+        ("deg", _ip),
     ]
 
 
@@ -140,11 +145,11 @@ def lib():
         L.ora_get_state.argtypes = [vp, ctypes.POINTER(State), ctypes.POINTER(Scalars)]
         L.ora_get_state.restype = ctypes.c_int
         L.ora_get_attempt.argtypes = [vp, ctypes.POINTER(AttemptOut)]
-        L.ora_attempt.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.ora_attempt.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.ora_attempt.restype = ctypes.c_int
-        L.ora_integrate.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.ora_integrate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.ora_integrate.restype = ctypes.c_int
-        L.ora_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64]
+        L.ora_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64]
         L.ora_run.restype = ctypes.c_double
         L.ora_search.argtypes = [vp]
         L.ora_remove.argtypes = [vp]
@@ -189,9 +194,10 @@ def consts_to_dict(c):
 
 
 def consts_from_dict(d):
+    """Consts from a dict; the flow keys default to zero (no flow box)."""
     c = Consts()
     for name, _ in Consts._fields_:
-        v = d[name]
+        v = d.get(name, (0.0, 0.0, 0.0)) if name.startswith("flow_") else d[name]
         if hasattr(v, "__len__"):
             arr = getattr(c, name)
             for k in range(3):
@@ -260,7 +266,7 @@ class Foam:
             "x": np.zeros((4, n)), "v": np.zeros((4, n)), "vo": np.zeros((4, n)),
             "saved": np.zeros((4, n)), "path": np.zeros(n), "err": np.zeros(n),
             "wc": np.zeros((3, n), np.int32), "idx": np.zeros(n, np.int32),
-            "nbr_off": np.zeros(n + 1, np.int32),
+            "nbr_off": np.zeros(n + 1, np.int32), "deg": np.zeros(n, np.int32),
         }
         nbr = np.zeros(max(1, 2 * sc["num_pairs"]), np.int32)
         s = State()
@@ -274,6 +280,7 @@ class Foam:
         s.path, s.err = _dptr(st["path"]), _dptr(st["err"])
         s.idx, s.nbr_off, s.nbr = _iptr(st["idx"]), _iptr(st["nbr_off"]), _iptr(nbr)
         s.nbr_cap = nbr.size
+        s.deg = _iptr(st["deg"])
         rc = lib().ora_get_state(self.h, ctypes.byref(s), None)
         if rc != 0:
             raise RuntimeError("oracle neighbour list larger than expected")
@@ -317,6 +324,8 @@ class Foam:
         idx, off, nbr = i(st["idx"]), i(st["nbr_off"]), i(st["nbr"] if len(st["nbr"]) else [0])
         s.idx, s.nbr_off, s.nbr = _iptr(idx), _iptr(off), _iptr(nbr)
         s.nbr_cap = len(st["nbr"])
+        if st.get("deg") is not None:
+            s.deg = _iptr(i(st["deg"]))
         sc = Scalars()
         for name, _ in Scalars._fields_:
             if name == "cell_dim":
@@ -340,14 +349,14 @@ class Foam:
         lib().ora_get_attempt(self.h, ctypes.byref(a))
         return o
 
-    def attempt(self, gas=True, path=True):
-        return lib().ora_attempt(self.h, int(gas), int(path))
+    def attempt(self, gas=True, path=True, flow=False):
+        return lib().ora_attempt(self.h, int(gas), int(path), int(flow))
 
-    def integrate(self, gas=True, path=True):
-        return lib().ora_integrate(self.h, int(gas), int(path))
+    def integrate(self, gas=True, path=True, flow=False):
+        return lib().ora_integrate(self.h, int(gas), int(path), int(flow))
 
-    def run(self, steps, gas=True, path=True):
-        return lib().ora_run(self.h, int(gas), int(path), int(steps))
+    def run(self, steps, gas=True, path=True, flow=False):
+        return lib().ora_run(self.h, int(gas), int(path), int(flow), int(steps))
 
     def search(self):
         lib().ora_search(self.h)

@@ -48,6 +48,10 @@ struct Foam {
     std::vector<double> x[4], v[4], vo[4], saved[4], path, err;
     std::vector<int32_t> wc[3], idx;
     std::vector<int32_t> nbr_off, nbr;
+    /* list length at the last search: the reference's numNeighbors
+     * (src/kernels.cu:713-719), which a removal moves with the bubble
+     * (src/kernels.cu:820-827) but does not recount */
+    std::vector<int32_t> deg;
     /* outputs of the last attempt */
     std::vector<double> xp[4], vp[4], ov, path_new;
     std::vector<uint8_t> del;
@@ -81,6 +85,26 @@ inline double sq_len(const double d[3]) {
     t += d[1] * d[1];
     t += d[2] * d[2];
     return t;
+}
+
+/* reference src/kernels.cu:497-544 (addFlowVelocity), in its own terms:
+ * multiplier = 1 / numNeighbors, the inside test per axis, and the term
+ * !inside * multiplier * flowV + flowVel * inside added to the predicted
+ * velocity (before the walls). */
+void add_flow_velocity(const ora_consts &c, double xi, double yi, double zi, double rp,
+                       int32_t num_neighbors, const double flow_v[3], double dxdtp[3]) {
+    double multiplier = num_neighbors;
+    multiplier = (multiplier > 0 ? 1.0 / multiplier : 0.0);
+    const double ri_sq = rp * rp;
+    const double x[3] = {xi, yi, zi};
+    int32_t inside = 1;
+    for (int k = 0; k < c.dim; ++k) {
+        inside *= (int32_t)((x[k] < c.flow_tfr[k] && x[k] > c.flow_lbb[k]) ||
+                            ((c.flow_lbb[k] - x[k]) * (c.flow_lbb[k] - x[k]) <= ri_sq) ||
+                            ((c.flow_tfr[k] - x[k]) * (c.flow_tfr[k] - x[k]) <= ri_sq));
+    }
+    for (int k = 0; k < c.dim; ++k)
+        dxdtp[k] += !inside * multiplier * flow_v[k] + c.flow_vel[k] * inside;
 }
 
 /* reference src/kernels.cu:546-593 (addWallVelocity) */
@@ -156,7 +180,7 @@ void resize_attempt(Foam &f) {
 /* One step() attempt: reference src/simulator.cu:192-249 with kernels
  * preIntegrate (src/kernels.cu:40-170), pairwiseInteraction (:172-337) and
  * postIntegrate (:339-495). */
-int attempt(Foam &f, bool gas, bool incr_path) {
+int attempt(Foam &f, bool gas, bool incr_path, bool flow) {
     const ora_consts &c = f.c;
     const int32_t n = f.n;
     const int D = c.dim;
@@ -186,6 +210,7 @@ int attempt(Foam &f, bool gas, bool incr_path) {
 #pragma omp parallel for reduction(+ : s_oa, s_oar) if (g_threads > 1) num_threads(g_threads) schedule(static)
     for (int32_t i = 0; i < n; ++i) {
         double acc[3] = {0.0, 0.0, 0.0};
+        double flow_v[3] = {0.0, 0.0, 0.0};
         double ovl = 0.0, dr = 0.0;
         const double ri = f.xp[3][i];
         for (int32_t q = f.nbr_off[i]; q < f.nbr_off[i + 1]; ++q) {
@@ -199,6 +224,11 @@ int attempt(Foam &f, bool gas, bool incr_path) {
             wrapped_diff(c, f.xp[0][i], f.xp[1][i], 3 == D ? f.xp[2][i] : 0.0, f.xp[0][j],
                          f.xp[1][j], 3 == D ? f.xp[2][j] : 0.0, d);
             const double m = sq_len(d);
+            /* flow velocities: the pair (a, b), a < b, adds v[a] into b for
+             * every listed pair (src/kernels.cu:191-208) but v[b] into a only
+             * past the overlap test (:281-307) */
+            if (flow && (i == b || radii * radii >= m))
+                for (int k = 0; k < D; ++k) flow_v[k] += f.v[k][j];
             if (radii * radii >= m) {
                 const double s = 1.0 / std::sqrt(m) - 1.0 / radii;
                 for (int k = 0; k < D; ++k) acc[k] += d[k] * c.f0 * s;
@@ -225,6 +255,9 @@ int attempt(Foam &f, bool gas, bool incr_path) {
                 }
             }
         }
+        if (flow)
+            add_flow_velocity(c, f.xp[0][i], f.xp[1][i], 3 == D ? f.xp[2][i] : 0.0, f.xp[3][i],
+                              f.deg[i], flow_v, acc);
         for (int k = 0; k < D; ++k) f.vp[k][i] = acc[k];
         f.vp[3][i] = dr;
         f.ov[i] = ovl;
@@ -504,6 +537,7 @@ void search(Foam &f) {
         }
     }
     s.num_pairs = f.nbr_off[n] / 2;
+    f.deg.assign(counts.begin(), counts.end());
 }
 
 /* removeBubbles: src/simulator.cu:180-190 with swapDataCountPairs
@@ -541,6 +575,7 @@ void remove_bubbles(Foam &f) {
         f.err[dst] = f.err[src];
         for (int k = 0; k < 3; ++k) f.wc[k][dst] = f.wc[k][src];
         f.idx[dst] = f.idx[src];
+        f.deg[dst] = f.deg[src];
         lists[dst].assign(f.nbr.begin() + f.nbr_off[src], f.nbr.begin() + f.nbr_off[src + 1]);
         newidx[src] = dst;
     }
@@ -567,6 +602,7 @@ void remove_bubbles(Foam &f) {
     f.err.resize(nnew);
     for (int k = 0; k < 3; ++k) f.wc[k].resize(nnew);
     f.idx.resize(nnew);
+    f.deg.resize(nnew);
     /* volume fix: r *= M / S_V^(1/D), src/kernels.cu:861-867, :932-934 */
     double vm = 3 == c.dim ? 1.0 / det_cbrt(f.s.s_v) : 1.0 / std::sqrt(f.s.s_v);
     vm *= c.vol_mul;
@@ -578,10 +614,10 @@ void remove_bubbles(Foam &f) {
 }
 
 /* integrate(): src/simulator.cu:251-337 */
-int integrate(Foam &f, bool gas, bool incr_path) {
+int integrate(Foam &f, bool gas, bool incr_path, bool flow) {
     int tries = 0;
     do {
-        attempt(f, gas, incr_path);
+        attempt(f, gas, incr_path, flow);
         ++tries;
     } while (!f.s.accepted);
     const int need_search = f.s.need_search;
@@ -658,6 +694,7 @@ Foam *new_foam(const ora_consts &c, int32_t n) {
     for (int k = 0; k < 3; ++k) f->wc[k].assign(n, 0);
     f->idx.assign(n, 0);
     f->nbr_off.assign(n + 1, 0);
+    f->deg.assign(n, 0);
     f->s.count = n;
     return f;
 }
@@ -700,6 +737,9 @@ void ora_set_state(void *p, const ora_state *st, const ora_scalars *sc) {
     f.nbr.assign(f.nbr_off[n], 0);
     if (st->nbr && f.nbr_off[n] > 0) std::memcpy(f.nbr.data(), st->nbr, sizeof(int32_t) * f.nbr_off[n]);
     f.s.num_pairs = f.nbr_off[n] / 2;
+    f.deg.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i)
+        f.deg[i] = st->deg ? st->deg[i] : f.nbr_off[i + 1] - f.nbr_off[i];
 }
 
 int ora_get_state(void *p, ora_state *st, ora_scalars *sc) {
@@ -724,6 +764,7 @@ int ora_get_state(void *p, ora_state *st, ora_scalars *sc) {
         if (st->wc[k]) std::memcpy(st->wc[k], f.wc[k].data(), sizeof(int32_t) * n);
     if (st->idx) std::memcpy(st->idx, f.idx.data(), sizeof(int32_t) * n);
     if (st->nbr_off) std::memcpy(st->nbr_off, f.nbr_off.data(), sizeof(int32_t) * (n + 1));
+    if (st->deg) std::memcpy(st->deg, f.deg.data(), sizeof(int32_t) * n);
     if (st->nbr) {
         if ((int64_t)f.nbr.size() > st->nbr_cap) return -1;
         std::memcpy(st->nbr, f.nbr.data(), sizeof(int32_t) * f.nbr.size());
@@ -743,18 +784,18 @@ void ora_get_attempt(void *p, ora_attempt_out *o) {
     if (o->del) std::memcpy(o->del, f.del.data(), n);
 }
 
-int ora_attempt(void *p, int gas, int path) {
-    return attempt(*static_cast<Foam *>(p), gas != 0, path != 0);
+int ora_attempt(void *p, int gas, int path, int flow) {
+    return attempt(*static_cast<Foam *>(p), gas != 0, path != 0, flow != 0);
 }
 
-int ora_integrate(void *p, int gas, int path) {
-    return integrate(*static_cast<Foam *>(p), gas != 0, path != 0);
+int ora_integrate(void *p, int gas, int path, int flow) {
+    return integrate(*static_cast<Foam *>(p), gas != 0, path != 0, flow != 0);
 }
 
-double ora_run(void *p, int gas, int path, int64_t n) {
+double ora_run(void *p, int gas, int path, int flow, int64_t n) {
     Foam &f = *static_cast<Foam *>(p);
     const auto t0 = std::chrono::steady_clock::now();
-    for (int64_t k = 0; k < n; ++k) integrate(f, gas != 0, path != 0);
+    for (int64_t k = 0; k < n; ++k) integrate(f, gas != 0, path != 0, flow != 0);
     const auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }

@@ -36,6 +36,9 @@ typedef struct ora_consts {
     double vol_mul;         /* volume multiplier M */
     double tol;             /* errorTolerance */
     int32_t cell_cap;       /* maximum number of cells in the cell grid */
+    double flow_lbb[3];     /* imposed-flow box, box coordinates */
+    double flow_tfr[3];
+    double flow_vel[3];     /* imposed velocity (already times f0) */
 } ora_consts;
 
 typedef struct ora_scalars {
@@ -71,6 +74,8 @@ typedef struct ora_state {
     int32_t *nbr_off;       /* CSR offsets, count + 1 entries */
     int32_t *nbr;           /* CSR full neighbour lists (both directions) */
     int64_t nbr_cap;        /* capacity of nbr on get, length on set */
+    int32_t *deg;           /* list length at the last search (imposed flow);
+                               set: NULL = the current list lengths */
 } ora_state;
 
 typedef struct ora_attempt_out {
@@ -106,11 +111,11 @@ int ora_get_state(void *f, ora_state *s, ora_scalars *sc);
 void ora_get_attempt(void *f, ora_attempt_out *o);
 
 /* One step() attempt (predict, pair, correct, dt control); no commit. */
-int ora_attempt(void *f, int gas, int path);
+int ora_attempt(void *f, int gas, int path, int flow);
 /* integrate(): attempts until accepted, commit, removal, search. */
-int ora_integrate(void *f, int gas, int path);
+int ora_integrate(void *f, int gas, int path, int flow);
 /* integrate() repeated n times; returns seconds of wall time. */
-double ora_run(void *f, int gas, int path, int64_t n);
+double ora_run(void *f, int gas, int path, int flow, int64_t n);
 void ora_search(void *f);
 void ora_remove(void *f);
 /* O(N^2) pair set (i < j) from the current state, ascending. */

@@ -41,7 +41,7 @@ def oracle_from_sim(ora, sim):
     st, sc = sim.get_state()
     consts = {k: sc[k] for k in ("dim", "wall", "lbb", "tfr", "interval", "wall_drag", "skin",
                                  "min_rad", "f0", "k_param", "kappa", "avg_area_in", "vol_mul",
-                                 "tol", "cell_cap")}
+                                 "tol", "cell_cap", "flow_lbb", "flow_tfr", "flow_vel")}
     osc = {k: sc[k] for k in ("count", "cell_dim", "dt", "time_int", "time_frac", "max_r_cell",
                               "steps", "attempts", "searches", "removals", "bubble_steps")}
     for k in ("s_a", "s_ar", "s_oa", "s_oar", "s_v", "max_err", "max_r", "max_exp"):

@@ -68,10 +68,17 @@ def test_json_schema_reader():
     assert c3.seed == 7 and c3.phi == 0.84
 
 
+def test_json_flow_keys():
+    # flow.impose only: the other flow keys keep input_parameters.json's values
+    c = ca.Config.from_json(os.path.join(GOLDEN, "input_flow.json"))
+    assert c.flow == 1
+    assert c.flow_vel == (0.01, 0.0, 0.0)
+    assert c.flow_lbb == (0.0, 0.475, 0.0) and c.flow_tfr == (0.4, 0.525, 0.0)
+    c = ca.Config.from_json(os.path.join(GOLDEN, "input_small_2d.json"))
+    assert c.flow == 0
+
+
 def test_json_errors_are_reported():
-    with pytest.raises(ca.CubbleError) as e:
-        ca.Config.from_json(os.path.join(GOLDEN, "input_flow.json"))
-    assert e.value.code == ca.E_UNSUPPORTED
     with pytest.raises(ca.CubbleError) as e:
         ca.Config.from_json(os.path.join(GOLDEN, "does_not_exist.json"))
     assert e.value.code == ca.E_IO

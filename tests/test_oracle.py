@@ -24,12 +24,14 @@ PI = 3.1415926535897932384626433832795028841971693993
 
 
 def consts(dim, L=20.0, wall=(0, 0, 0), tol=1e-4, f0=1.0, kappa=0.01, k=0.1, ain=1.0,
-           vol_mul=1.0, skin=0.3, min_rad=0.1, drag=0.1, cell_cap=1 << 20):
+           vol_mul=1.0, skin=0.3, min_rad=0.1, drag=0.1, cell_cap=1 << 20,
+           flow_lbb=(0.0, 0.0, 0.0), flow_tfr=(0.0, 0.0, 0.0), flow_vel=(0.0, 0.0, 0.0)):
     iv = [L, L, L if dim == 3 else 0.0]
     return {
         "dim": dim, "wall": list(wall), "lbb": [0.0, 0.0, 0.0], "tfr": iv, "interval": iv,
         "wall_drag": drag, "skin": skin, "min_rad": min_rad, "f0": f0, "k_param": k,
         "kappa": kappa, "avg_area_in": ain, "vol_mul": vol_mul, "tol": tol, "cell_cap": cell_cap,
+        "flow_lbb": list(flow_lbb), "flow_tfr": list(flow_tfr), "flow_vel": list(flow_vel),
     }
 
 
@@ -374,6 +376,12 @@ def test_removal_compaction_and_volume_fix(ora, dim, n, frac):
     st_pre, _ = f.get_state()
     f.remove()
     st2, sc2 = f.get_state()
+    # the imposed flow's divisor moves with its bubble and is not reduced by
+    # the dropped pairs (the reference's numNeighbors, src/kernels.cu:820-827)
+    deg_before = dict(zip(idx_before.tolist(), st_pre["deg"].tolist()))
+    assert [deg_before[i] for i in st2["idx"].tolist()] == st2["deg"].tolist()
+    assert np.all(st2["deg"] >= np.diff(st2["nbr_off"]))
+    assert np.any(st2["deg"] > np.diff(st2["nbr_off"]))
     nnew = n - nd
     assert sc2["count"] == nnew
     survivors = idx_before[~dele]
@@ -517,3 +525,67 @@ def test_observe_matches_brute_force_minimum_image(ora, dim, n, phi, seed):
     for q in range(4):
         want = st["v"][q].sum() if (q != 2 or dim == 3) else 0.0
         assert sums[4 + q] == pytest.approx(want, rel=1e-12, abs=1e-12 * np.abs(st["v"][q]).sum())
+
+
+# --------------------------------------------------------------------------
+# imposed flow
+# --------------------------------------------------------------------------
+def test_flow_closed_forms(ora):
+    """Velocities are set, positions are fixed apart: listed pairs that do not
+    overlap exert no force, so the predicted velocity is the flow term alone.
+    Inside the box the bubble takes the imposed velocity; outside, the sum of
+    the velocities of listed neighbours with a smaller index and of
+    overlapping ones with a larger index, over its list length."""
+    vel = (0.3, -0.2)
+    pos = [(4.0, 4.0),                            # 0: inside the flow box
+           (6.3, 15.0 - 11.0),                    # 1: 0.3 past a face, r 0.5: inside
+           (10.0, 15.0), (11.1, 15.0), (12.2, 15.0),  # 2-4: a chain 1.1 apart
+           (15.0, 5.0), (15.9, 5.0)]              # 5-6: overlapping
+    r = [0.5] * 7
+    v = np.zeros((4, 7))
+    v[0, :] = [5.0, 6.0, 1.0, 10.0, 100.0, 7.0, 70.0]
+    v[1, :] = [-5.0, -6.0, 2.0, 20.0, 200.0, 8.0, 80.0]
+    c = consts(2, flow_lbb=(2.0, 2.0, 0.0), flow_tfr=(6.0, 6.0, 0.0), flow_vel=vel + (0.0,))
+    st = make_state(pos, r, 2, v, v)
+    # the lists a search gives (listed: within r_i + r_j + skin = 1.3), set
+    # by hand so that the indices stay as written
+    lists = [[], [], [3], [2, 4], [3], [6], [5]]
+    st["nbr_off"] = np.cumsum([0] + [len(x) for x in lists]).astype(np.int32)
+    st["nbr"] = np.array(sum(lists, []), np.int32)
+
+    def run(flow):
+        f = ora.Foam.from_state(c, st, scalars(7))
+        f.attempt(gas=False, path=False, flow=flow)
+        return f, f.attempt_out()["vp"]
+
+    f, vp = run(True)
+    assert f.get_state()[0]["deg"].tolist() == [0, 0, 1, 2, 1, 1, 1]
+    _, vp0 = run(False)
+    assert np.all(vp0[:2, :5] == 0.0)            # no overlaps among 0-4
+    for i in (0, 1):
+        assert tuple(vp[:2, i]) == vel
+    assert tuple(vp[:2, 2]) == (0.0, 0.0)        # its neighbour 3 is larger, apart
+    assert tuple(vp[:2, 3]) == (0.5, 1.0)        # v2 / 2 (4 is larger, apart)
+    assert tuple(vp[:2, 4]) == (10.0, 20.0)      # v3 / 1
+    # overlapping: each takes the other's velocity, on top of the pair force
+    assert vp[0, 5] - vp0[0, 5] == pytest.approx(70.0, rel=1e-14)
+    assert vp[1, 5] - vp0[1, 5] == pytest.approx(80.0, rel=1e-14)
+    assert vp[0, 6] - vp0[0, 6] == pytest.approx(7.0, rel=1e-14)
+    assert vp[1, 6] - vp0[1, 6] == pytest.approx(8.0, rel=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_flow_box_edges_count_within_radius(ora, dim):
+    """One free bubble per position: inside strictly between the faces or
+    within its radius of a face, on every axis."""
+    lo, hi, rr = 5.0, 9.0, 0.5
+    vel = (0.25, 0.5, -0.75)
+    cases = [(7.0, True), (9.4, True), (9.5, True), (9.6, False), (4.6, True), (4.4, False)]
+    for x, inside in cases:
+        p = [(x, 7.0, 7.0)[:dim]]
+        kw = dict(flow_lbb=(lo,) * 3, flow_tfr=(hi,) * 3, flow_vel=vel, dt=1e-4)
+        f = foam(ora, p, [rr], dim, **kw)
+        f.attempt(gas=False, path=False, flow=True)
+        vp = f.attempt_out()["vp"]
+        want = vel[:dim] if inside else (0.0,) * dim
+        assert tuple(vp[:dim, 0]) == want, (x, inside)
