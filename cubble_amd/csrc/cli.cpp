@@ -1,8 +1,12 @@
 // cli.cpp -- drop-in command line entry, cubble_run(argc, argv), and the
 // one-line main of cubble-cli.  Mirrors cubble::run (reference
-// src/simulator.cu:1161-1186) and simulate() (src/simulator.cu:958-1157):
-// argument check, device check, the integration loop and its stop rule, the
-// printed table, results.dat and the snapshot CSVs.
+// src/simulator.cu:1161-1186), the stabilisation phase of init()
+// (src/simulator.cu:789-791, :848-955) and simulate()
+// (src/simulator.cu:958-1157): argument check, device check, stabilisation
+// with its table, the integration loop and its stop rule, the printed table,
+// results.dat and the snapshot CSVs.  The foam is the synthetic one drawn at
+// the target gas fraction; the reference's grid placement and box rescale
+// are not repeated.
 //
 // The reference's host loop checks after every accepted step whether to
 // print, snapshot or stop.  Here the steps run on the device
@@ -51,6 +55,70 @@ int fail(cubble_sim *sim) {
     std::fprintf(stderr, "error: %s\n", cubble_last_error(sim));
     cubble_destroy(sim);
     return EXIT_FAILURE;
+}
+
+// stabilize() (reference src/simulator.cu:339-360): `steps` accepted steps
+// without gas exchange, flow or path, with the energy before and after.
+int stabilize(cubble_sim *sim, int64_t steps, double &e1, double &e2) {
+    cubble_observables o;
+    int rc = cubble_observe(sim, &o, nullptr);
+    if (rc) return rc;
+    e1 = o.total_energy;
+    const cubble_integrate_params ip = {0, 0, 0};
+    cubble_step_result res;
+    if ((rc = cubble_run_steps(sim, &ip, steps, &res))) return rc;
+    if ((rc = cubble_observe(sim, &o, nullptr))) return rc;
+    e2 = o.total_energy;
+    return CUBBLE_OK;
+}
+
+// The stabilisation phase of init() (src/simulator.cu:789-791, :848-899):
+// five rounds, then rounds until the energy change per scaled time is below
+// maxDeltaEnergy (or small for a low energy), at most about 500, printing
+// the reference's table.  `searches0` returns the search count the
+// reference's counter starts the first integration row from (it is not
+// reset after the last round).
+int stabilization_phase(cubble_sim *sim, int64_t steps, double max_de, double time_scale,
+                        int64_t &searches0) {
+    double e1 = 0.0, e2 = 0.0;
+    int rc = CUBBLE_OK;
+    std::printf("Stabilizing a few rounds after creation\n");
+    for (int k = 0; k < 5; ++k)
+        if ((rc = stabilize(sim, steps, e1, e2))) return rc;
+    std::printf("\n=============\nStabilization\n=============\n");
+    std::printf("%-7s %-11s %-11s %-11s %-9s\n", "#steps", "dE", "e1", "e2", "#searches");
+    cubble_scalars sc;
+    if ((rc = cubble_get_scalars(sim, &sc))) return rc;
+    searches0 = sc.searches;
+    const int failsafe = 500;
+    for (int num = 0;; ++num) {
+        if ((rc = cubble_reset_clock(sim, 0)) || (rc = stabilize(sim, steps, e1, e2)) ||
+            (rc = cubble_get_scalars(sim, &sc)))
+            return rc;
+        const double time = ((double)sc.time_int + sc.time_frac) * time_scale;
+        double de = std::abs(e2 - e1);
+        if (de > 0.0) de *= 2.0 / ((e2 + e1) * time);
+        if (de < max_de || (e2 < 1.0 && de < 0.1)) {
+            std::printf("Final energies:");
+            std::printf("\nbefore: %9.5e", e1);
+            std::printf("\nafter: %9.5e", e2);
+            std::printf("\ndelta: %9.5e", de);
+            std::printf("\ntime: %9.5g\n", time);
+            break;
+        } else if (num > failsafe) {
+            std::printf("Over %d steps taken and required delta energy not reached. "
+                        "Constraints might be too strict.\n",
+                        num);
+            break;
+        }
+        std::printf("%-7d ", (int)((num + 1) * steps));
+        std::printf("%-9.5e ", de);
+        std::printf("%-9.5e ", e1);
+        std::printf("%-9.5e ", e2);
+        std::printf("%-9ld\n", (long)(sc.searches - searches0));
+        searches0 = sc.searches;
+    }
+    return CUBBLE_OK;
 }
 
 // Snapshot files <name>.csv.<k> (reference saveSnapshot, src/simulator.cu:387-512).
@@ -150,12 +218,13 @@ extern "C" int cubble_run(int argc, char **argv) {
     }
     cubble_config cfg;
     if (cubble_config_from_json(argv[1], &cfg) != CUBBLE_OK) return fail(nullptr);
-    double snap_freq = 0.0;
-    std::string snap_name, err;
-    if (cubble_amd::parse_snapshot_json(argv[1], snap_freq, snap_name, err) != CUBBLE_OK) {
+    cubble_amd::RunSettings rs;
+    std::string err;
+    if (cubble_amd::parse_run_json(argv[1], rs, err) != CUBBLE_OK) {
         std::fprintf(stderr, "error: %s\n", err.c_str());
         return EXIT_FAILURE;
     }
+    const double snap_freq = rs.snapshot_frequency;
     cubble_sim *sim = nullptr;
     if (cubble_create(&cfg, &sim) != CUBBLE_OK) return fail(nullptr);
     int rc = CUBBLE_OK;
@@ -167,11 +236,20 @@ extern "C" int cubble_run(int argc, char **argv) {
         0.5 * (3 == sc.dim ? std::fmin(L[0], std::fmin(L[1], L[2])) : std::fmin(L[0], L[1]));
     // timeScalingFactor = K / avgRad^2 (src/simulator.cu:557-559)
     const double time_scale = cfg.k_param / (cfg.r_mean * cfg.r_mean);
-    // energy of the starting state, errors cleared (src/simulator.cu:947-951)
-    if (cubble_reset_print_window(sim) || cubble_observe(sim, &obs, nullptr)) return fail(sim);
+    int64_t row_searches0 = sc.searches;
+    if (rs.stabilization_steps > 0 &&
+        stabilization_phase(sim, rs.stabilization_steps, rs.max_delta_energy, time_scale,
+                            row_searches0))
+        return fail(sim);
+    // integration starts from zero time, wrap counters and errors, with the
+    // energy of the stabilised foam (src/simulator.cu:938-955)
+    if (cubble_reset_clock(sim, 1) || cubble_reset_print_window(sim) ||
+        cubble_observe(sim, &obs, nullptr) || cubble_get_scalars(sim, &sc))
+        return fail(sim);
     double e1 = obs.total_energy;
+    const int64_t steps0 = sc.steps;  // run.maxSteps counts integration steps
 
-    Snapshots snaps(snap_name);
+    Snapshots snaps(rs.snapshot_name);
     int64_t num_snaps = 0;
     if (snap_freq > 0.0) {
         if (snaps.start(sim, sc) || snaps.write(sim, sc)) return fail(sim);
@@ -185,7 +263,7 @@ extern "C" int cubble_run(int argc, char **argv) {
     const cubble_integrate_params ip = {1, cfg.flow, 1};
     cubble_step_result res;
     int64_t printed = 1;
-    int64_t row_steps0 = sc.steps, row_searches0 = sc.searches;
+    int64_t row_steps0 = sc.steps;
     // the reference counts the first row's steps from 0 at its first step
     // (the counter is incremented after the print check, src/simulator.cu:1134),
     // so its first row reports one step fewer and averages over that count
@@ -199,12 +277,12 @@ extern "C" int cubble_run(int argc, char **argv) {
         }
         const cubble_stop_rules rules = {cfg.num_end, 1, min_interval, next.i, next.f};
         int64_t chunk = kChunk;
-        if (cfg.max_steps > 0) chunk = std::min<int64_t>(chunk, cfg.max_steps - sc.steps);
+        if (cfg.max_steps > 0) chunk = std::min<int64_t>(chunk, cfg.max_steps - (sc.steps - steps0));
         if ((rc = cubble_set_stop_rules(sim, &rules)) ||
             (rc = cubble_run_steps(sim, &ip, chunk, &res)) || (rc = cubble_get_scalars(sim, &sc)))
             break;
         const bool more = sc.count > cfg.num_end && sc.max_r < min_interval &&
-                          (cfg.max_steps <= 0 || sc.steps < cfg.max_steps);
+                          (cfg.max_steps <= 0 || sc.steps - steps0 < cfg.max_steps);
         const bool print = reached(sc, t_print);
         if (print) {
             if ((rc = cubble_observe(sim, &obs, nullptr))) break;
@@ -248,8 +326,9 @@ extern "C" int cubble_run(int argc, char **argv) {
         std::printf("Stopping simulation, since the radius of the largest bubble (%g) is greater "
                     "than the simulation box (%g, %g, %g)\n",
                     sc.max_r, L[0], L[1], L[2]);
-    else if (cfg.max_steps > 0 && sc.steps >= cfg.max_steps)
-        std::printf("Stopping simulation after %lld steps (run.maxSteps)\n", (long long)sc.steps);
+    else if (cfg.max_steps > 0 && sc.steps - steps0 >= cfg.max_steps)
+        std::printf("Stopping simulation after %lld steps (run.maxSteps)\n",
+                    (long long)(sc.steps - steps0));
     else
         std::printf("Stopping simulation for an unknown reason...\n");
     if (snap_freq > 0.0 && snaps.write(sim, sc)) return fail(sim);

@@ -451,10 +451,8 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
     return CUBBLE_OK;
 }
 
-int parse_snapshot_json(const char *path, double &frequency, std::string &filename,
-                        std::string &err) {
-    frequency = 0.0;
-    filename = "snapshot";
+int parse_run_json(const char *path, RunSettings &rs, std::string &err) {
+    rs = RunSettings();
     std::ifstream f(path);
     if (!f.is_open()) {
         err = std::string("couldn't open input file ") + (path ? path : "(null)");
@@ -470,8 +468,14 @@ int parse_snapshot_json(const char *path, double &frequency, std::string &filena
         return CUBBLE_E_IO;
     }
     double v = 0.0;
-    if (jnum(root, {"snapShot", "frequency"}, v)) frequency = v;
-    jstr(root, {"snapShot", "filename"}, filename);
+    if (jnum(root, {"snapShot", "frequency"}, v)) rs.snapshot_frequency = v;
+    jstr(root, {"snapShot", "filename"}, rs.snapshot_name);
+    if (jnum(root, {"stabilization", "steps"}, v)) rs.stabilization_steps = (int64_t)v;
+    if (jnum(root, {"stabilization", "maxDeltaEnergy"}, v)) rs.max_delta_energy = v;
+    if (rs.stabilization_steps < 0) {
+        err = "stabilization.steps must be >= 0";
+        return CUBBLE_E_INVALID;
+    }
     return CUBBLE_OK;
 }
 

@@ -27,10 +27,17 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
 // input_parameters.json (reference schema) -> config.
 int parse_config_json(const char *path, cubble_config &cfg, std::string &err);
 
-// snapShot.frequency and snapShot.filename of input_parameters.json
-// (reference src/simulator.cu:568-569); frequency 0 when absent.
-int parse_snapshot_json(const char *path, double &frequency, std::string &filename,
-                        std::string &err);
+// Settings of the command line's run that cubble_config does not hold:
+// snapShot.{frequency, filename} and stabilization.{steps, maxDeltaEnergy}
+// of input_parameters.json (reference src/simulator.cu:568-569, :544,
+// :875).  Absent keys: no snapshots, no stabilisation phase.
+struct RunSettings {
+    double snapshot_frequency = 0.0;
+    std::string snapshot_name = "snapshot";
+    int64_t stabilization_steps = 0;
+    double max_delta_energy = 0.0;
+};
+int parse_run_json(const char *path, RunSettings &rs, std::string &err);
 
 void config_default(cubble_config &cfg);
 

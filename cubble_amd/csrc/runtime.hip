@@ -386,6 +386,20 @@ int cubble_reset_print_window(cubble_sim *s) {
     return e.write_dyn(0, h.d);
 }
 
+int cubble_reset_clock(cubble_sim *s, int32_t clear_wraps) {
+    if (!s) return CUBBLE_E_INVALID;
+    Engine &e = s->e;
+    Foam h;
+    int rc = e.read_header(0, h);
+    if (rc) return rc;
+    if (clear_wraps)
+        for (int q = 0; q < h.c.dim; ++q)
+            R_TRY(e, hipMemsetAsync(h.r.wc[q], 0, sizeof(int32_t) * (size_t)h.cap, e.stream));
+    h.d.time_int = 0;
+    h.d.time_frac = 0.0;
+    return e.write_dyn(0, h.d);
+}
+
 int cubble_time_kernels(cubble_sim *s, const cubble_integrate_params *ip, int32_t reps,
                         cubble_kernel_times *out) {
     return s ? time_kernels(s->e, ip, reps, out) : CUBBLE_E_INVALID;

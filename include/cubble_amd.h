@@ -259,6 +259,12 @@ int cubble_set_stop_rules(cubble_sim *sim, const cubble_stop_rules *rules);
  * cubble_scalars.ts_min / ts_max / ts_sum report. */
 int cubble_reset_print_window(cubble_sim *sim);
 
+/* What the reference's init resets before each stabilisation round and
+ * before the integration loop (src/simulator.cu:866-867, :938-955): the
+ * clock to zero and, with clear_wraps, the wrap counters, so that unwrapped
+ * positions and snapshot displacements start from the stabilised foam. */
+int cubble_reset_clock(cubble_sim *sim, int32_t clear_wraps);
+
 /* ---- state access (tests, checkpoints) ----------------------------------- */
 int cubble_get_scalars(cubble_sim *sim, cubble_scalars *out);
 int cubble_get_state(cubble_sim *sim, cubble_state_view *view);

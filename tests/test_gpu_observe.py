@@ -140,19 +140,41 @@ def _g(x):
     return "%g" % x
 
 
+def _stabilize(sim, steps):
+    e1 = sim.observe()["total_energy"]
+    sim.run_steps(steps, ca.IntegrationParams(False, False, False))
+    return e1, sim.observe()["total_energy"]
+
+
 def _replay(json_path, rows):
-    """The reference's loop, one accepted step at a time from Python: the
-    first `rows` printed rows as (table text, results.dat line)."""
+    """The reference's init stabilisation and loop from Python, the loop one
+    accepted step at a time: the stabilisation lines, then the first `rows`
+    printed rows as (table text, results.dat line).  The stabilisation must
+    stop after its first counted round."""
     with open(json_path) as fh:
         j = json.load(fh)
     r_mean = j["bubbles"]["radius"]["mean"]
     num_end = int(j["bubbles"]["numEnd"])
     tsf = j["constants"]["K"]["value"] / (r_mean * r_mean)
+    steps = int(j["stabilization"]["steps"])
     sim = ca.Simulator(json_path=json_path)
+    for _ in range(5):
+        _stabilize(sim, steps)
+    searches0 = sim.scalars()["searches"]
+    sim.reset_clock()
+    e1, e2 = _stabilize(sim, steps)
+    sc = sim.scalars()
+    time = (sc["time_int"] + sc["time_frac"]) * tsf
+    de = abs(e2 - e1)
+    if de > 0:
+        de *= 2.0 / ((e2 + e1) * time)
+    assert de < j["stabilization"]["maxDeltaEnergy"]
+    stab = ["Final energies:", "before: %9.5e" % e1, "after: %9.5e" % e2, "delta: %9.5e" % de,
+            "time: %9.5g" % time]
+    sim.reset_clock(clear_wraps=True)
     sim.reset_print_window()
     e1 = sim.observe()["total_energy"]
     printed, n_steps, ts_min, ts_max, ts_sum = 1, 0, 9999999.9, -1.0, 0.0
-    searches0 = sim.scalars()["searches"]
     table, results = [], []
     while printed <= rows:
         r = sim.integrate(GAS)
@@ -184,7 +206,7 @@ def _replay(json_path, rows):
             sim.reset_print_window()
         n_steps += 1
     sim.close()
-    return table, results
+    return stab, table, results
 
 
 def test_cli_table_results_and_snapshots(tmp_path, monkeypatch, capfd):
@@ -192,10 +214,11 @@ def test_cli_table_results_and_snapshots(tmp_path, monkeypatch, capfd):
     with open(src) as fh:
         j = json.load(fh)
     # print every 0.01 time units (about 600 steps), a snapshot every 0.005,
-    # stop at 2500 steps
+    # stop at 2500 steps; stabilisation of 5 + 1 rounds of 30 steps
     j["constants"]["K"]["value"] = 100.0
     j["snapShot"] = {"filename": "bubbles", "frequency": 2.0}
     j["run"]["maxSteps"] = 2500
+    j["stabilization"] = {"steps": 30, "maxDeltaEnergy": 1e9}
     path = str(tmp_path / "input.json")
     with open(path, "w") as fh:
         json.dump(j, fh)
@@ -205,6 +228,8 @@ def test_cli_table_results_and_snapshots(tmp_path, monkeypatch, capfd):
     assert ca.run_cli(["cubble-cli", path]) == 0
     ctypes.CDLL(None).fflush(None)  # the library prints through C stdio
     out = capfd.readouterr().out.splitlines()
+    st = out.index("Stabilization") + 2
+    assert out[st] == "%-7s %-11s %-11s %-11s %-9s" % ("#steps", "dE", "e1", "e2", "#searches")
     head = out.index("Integration") + 2
     assert out[head] == "%-5s %-8s %-6s %-11s %9s %10s %-6s %-9s %-11s %-11s %-11s " % (
         "T", "phi", "R", "dE", "#b   ", "#pairs", "#steps", "#searches", "min ts", "max ts",
@@ -224,7 +249,8 @@ def test_cli_table_results_and_snapshots(tmp_path, monkeypatch, capfd):
         assert len(f) == 11 and int(f[0]) == k + 1
     # the first rows, replayed one step at a time, print the same text
     n = 2
-    table, results = _replay(path, n)
+    stab, table, results = _replay(path, n)
+    assert out[st + 1:st + 6] == stab
     assert rows[:n] == table
     assert res[:n] == results
     # snapshots: 0 at the start, one per 0.005 time units, one at the end
@@ -242,3 +268,18 @@ def test_cli_table_results_and_snapshots(tmp_path, monkeypatch, capfd):
     assert int(res[-1].split(" ")[2]) >= len(last)
     assert np.all(np.diff(np.sort(last["index"])) > 0)
     assert np.all(last["displacement"] >= 0)
+
+
+def test_reset_clock():
+    sim = ca.Simulator(config(2, 3000, seed=22))
+    sim.run_steps(200, GAS)
+    st, sc = sim.get_state()
+    assert sc["time_int"] + sc["time_frac"] > 0
+    sim.reset_clock()
+    st2, sc2 = sim.get_state()
+    assert sc2["time_int"] == 0 and sc2["time_frac"] == 0.0
+    assert np.array_equal(st2["wc"], st["wc"]) and np.array_equal(st2["x"], st["x"])
+    assert sc2["steps"] == sc["steps"] and sc2["dt"] == sc["dt"]
+    sim.reset_clock(clear_wraps=True)
+    assert not np.any(sim.get_state()[0]["wc"])
+    sim.close()
