@@ -225,25 +225,35 @@ __device__ __forceinline__ void for_neighbours(const int *__restrict__ nbr_,
                                                int m_nn, Body &&body) {
     const auto nbr = gmem(nbr_);
     const auto p4 = gmem(p4_);
-    for (int q0 = 0; q0 < m_nn; q0 += kGather) {
-        int js[kGather];
-        Rec pj[kGather];
-        // every load is issued (past the end: the last entry, in bounds), then
-        // the entries past the end are masked off
+    if (m_nn <= 0) return;
+    // list entries of batch q0 (past the end: the last entry, in bounds; the
+    // entries past the end are masked off)
+    auto load_ids = [&](int q0, int js[kGather]) {
 #pragma unroll
         for (int u = 0; u < kGather; ++u) {
             const int q = q0 + u < m_nn ? q0 + u : m_nn - 1;
             const int j = nbr[(size_t)q * cap + i];
             js[u] = q0 + u < m_nn ? j : -1;
         }
+    };
+    // software pipelined: the next batch's list entries are loaded while the
+    // current batch's records are in flight
+    int js[kGather];
+    load_ids(0, js);
+    for (int q0 = 0; q0 < m_nn; q0 += kGather) {
+        Rec pj[kGather];
 #pragma unroll
         for (int u = 0; u < kGather; ++u) {
             const int j = js[u] >= 0 ? js[u] : i;
             pj[u] = load_rec(p4 + j);
         }
+        int cur[kGather];
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) cur[u] = js[u];
+        if (q0 + kGather < m_nn) load_ids(q0 + kGather, js);
 #pragma unroll
         for (int u = 0; u < kGather; ++u)
-            if (js[u] >= 0) body(js[u], pj[u]);
+            if (cur[u] >= 0) body(cur[u], pj[u]);
     }
 }
 
