@@ -510,10 +510,11 @@ void launch_sums(const Grid &g) {
     hipLaunchKernelGGL(k_sums, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
 }
 
-void launch_correct(const Grid &g, uint32_t flags) {
+void launch_correct(const Grid &g, uint32_t flags, bool control) {
     hipLaunchKernelGGL(k_correct, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub,
                        flags);
-    hipLaunchKernelGGL(k_control, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
+    if (control)
+        hipLaunchKernelGGL(k_control, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
 }
 
 }  // namespace cubble_amd

@@ -405,11 +405,11 @@ int64_t cubble_foam_footprint(const cubble_config *cfg);
 
 /* ---- measurement ---------------------------------------------------------- */
 
-/* Average device time of each kernel of one attempt, measured with HIP
- * events recorded around each launch on the handle's stream over `reps`
- * non-committing attempts (predictor, pair, corrector) from the current
- * state.  No counterpart in the reference (it has NVTX ranges only,
- * src/simulator.cu:40-363); used by the benchmark's roofline report. */
+/* Average device time of each kernel of one attempt from the current state,
+ * measured with HIP events on the handle's stream around `reps` back-to-back
+ * launches of each (predictor; pair kernel; sums + corrector + controller,
+ * not committing).  No counterpart in the reference (it has NVTX ranges
+ * only, src/simulator.cu:40-363); used by the benchmark's roofline report. */
 typedef struct cubble_kernel_times {
     double predict_ms;
     double pair_ms;

@@ -283,3 +283,29 @@ def test_reset_clock():
     sim.reset_clock(clear_wraps=True)
     assert not np.any(sim.get_state()[0]["wc"])
     sim.close()
+
+
+@pytest.mark.parametrize("precision", [ca.PREC_FP64, ca.PREC_MIXED])
+def test_time_kernels_leaves_the_run_unchanged(precision):
+    """The benchmark's kernel timer re-runs the attempt kernels back to back;
+    the state, the counters and the rest of the run stay those of an untimed
+    twin (only the per-bubble error, a diagnostic maximum, may grow)."""
+    cfg = config(3, 20000, seed=23, precision=precision)
+    sim, twin = ca.Simulator(cfg), ca.Simulator(cfg)
+    sim.run_steps(20, GAS)
+    twin.run_steps(20, GAS)
+    st, sc = sim.get_state()
+    t = sim.time_kernels(5, GAS)
+    assert t["reps"] == 5 and min(t["predict_ms"], t["pair_ms"], t["correct_ms"]) > 0
+    st2, sc2 = sim.get_state()
+    for k in ("dt", "steps", "attempts", "time_int", "time_frac", "count", "searches"):
+        assert sc2[k] == sc[k], k
+    for k in ("x", "v", "vo", "saved", "nbr", "nbr_off", "idx"):
+        assert np.array_equal(st2[k], st[k]), k
+    sim.run_steps(30, GAS)
+    twin.run_steps(30, GAS)
+    a, b = sim.get_state()[0], twin.get_state()[0]
+    for k in ("x", "v", "nbr", "idx"):
+        assert np.array_equal(a[k], b[k]), k
+    sim.close()
+    twin.close()
