@@ -469,7 +469,7 @@ void Engine::enqueue_attempt(uint32_t flags) {
     if (CUBBLE_PREC_MIXED == precision) launch_pair_mixed(g, flags);
     else launch_pair(g, flags);
     launch_sums(g);
-    launch_correct(g, flags);
+    launch_correct(g, flags, CUBBLE_PREC_MIXED == precision);
 }
 
 void Engine::enqueue_group(uint32_t flags) {

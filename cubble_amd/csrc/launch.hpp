@@ -26,8 +26,9 @@ void launch_pair(const Grid &g, uint32_t flags);
 void launch_pair_mixed(const Grid &g, uint32_t flags);
 // S_A, S_Ar, S_OA, S_OAr from the predictor and pair partials (one block per foam)
 void launch_sums(const Grid &g);
-// corrector, then (control) the dt controller / commit (one block per foam)
-void launch_correct(const Grid &g, uint32_t flags, bool control = true);
+// corrector (mixed: the mixed-precision variant, for engines of that
+// precision), then (control) the dt controller / commit (one block per foam)
+void launch_correct(const Grid &g, uint32_t flags, bool mixed, bool control = true);
 
 // removeBubbles: emit the sorted deletion list and plan the moves, then move
 // the tail survivors into the holes, filter / remap the neighbour lists and

@@ -132,7 +132,7 @@ int time_kernels(Engine &e, const cubble_integrate_params *ip, int reps, cubble_
     // the corrector alone (its inputs: the sums, reduced once before)
     launch_sums(g);
     R_TRY(e, hipEventRecord(x[3], e.stream));
-    for (int r = 0; r < reps; ++r) launch_correct(g, flags, false);
+    for (int r = 0; r < reps; ++r) launch_correct(g, flags, CUBBLE_PREC_MIXED == e.precision, false);
     R_TRY(e, hipEventRecord(x[4], e.stream));
     R_TRY(e, hipStreamSynchronize(e.stream));
     // intervals [x0, x1) predict, [x1, x2) pair, [x3, x4) corrector

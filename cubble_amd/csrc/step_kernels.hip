@@ -427,25 +427,23 @@ __device__ void control(Foam *F, uint32_t flags, const CorrAcc &a) {
     if (stop_reached(d)) d.halt = 1;
 }
 
-template <int D>
+template <int D, bool M>
 __device__ __forceinline__ void correct_dispatch(const Consts &c, const Roles &R, const Scratch &S,
                                                  const Dyn &dyn, int base, int ipt, int n,
                                                  uint32_t flags, CorrAcc &a) {
     const bool gas = (flags & kGas) != 0, path = (flags & kPath) != 0;
-    auto run = [&](auto mixed) {
-        constexpr bool M = decltype(mixed)::value;
-        if (gas) {
-            if (path) correct_range<D, true, true, M>(c, R, S, dyn, base, ipt, n, a);
-            else correct_range<D, true, false, M>(c, R, S, dyn, base, ipt, n, a);
-        } else {
-            if (path) correct_range<D, false, true, M>(c, R, S, dyn, base, ipt, n, a);
-            else correct_range<D, false, false, M>(c, R, S, dyn, base, ipt, n, a);
-        }
-    };
-    if (S.q4) run(std::true_type{});
-    else run(std::false_type{});
+    if (gas) {
+        if (path) correct_range<D, true, true, M>(c, R, S, dyn, base, ipt, n, a);
+        else correct_range<D, true, false, M>(c, R, S, dyn, base, ipt, n, a);
+    } else {
+        if (path) correct_range<D, false, true, M>(c, R, S, dyn, base, ipt, n, a);
+        else correct_range<D, false, false, M>(c, R, S, dyn, base, ipt, n, a);
+    }
 }
 
+// One kernel per precision (MIXED: FP32 velocity buffers, recomputed
+// prediction), so that neither pays the other's registers.
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt, uint32_t flags) {
     const Ctx cx = bubble_ctx(foams, bt);
     Foam *F = cx.F;
@@ -457,8 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt,
     const Dyn dyn = F->d;
     const int base = cx.lb * kBlock * ipt;
     CorrAcc a{0.0, 0.0, 0.0, 0.0, 0};
-    if (3 == c.dim) correct_dispatch<3>(c, R, S, dyn, base, ipt, n, flags, a);
-    else correct_dispatch<2>(c, R, S, dyn, base, ipt, n, flags, a);
+    if (3 == c.dim) correct_dispatch<3, MIXED>(c, R, S, dyn, base, ipt, n, flags, a);
+    else correct_dispatch<2, MIXED>(c, R, S, dyn, base, ipt, n, flags, a);
     const double s_v = block_sum(a.s_v);
     const double me = block_max(a.max_err);
     const double mr = block_max(a.max_r);
@@ -510,9 +508,13 @@ void launch_sums(const Grid &g) {
     hipLaunchKernelGGL(k_sums, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
 }
 
-void launch_correct(const Grid &g, uint32_t flags, bool control) {
-    hipLaunchKernelGGL(k_correct, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub,
-                       flags);
+void launch_correct(const Grid &g, uint32_t flags, bool mixed, bool control) {
+    if (mixed)
+        hipLaunchKernelGGL(k_correct<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
+                           g.bt_bub, flags);
+    else
+        hipLaunchKernelGGL(k_correct<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
+                           g.bt_bub, flags);
     if (control)
         hipLaunchKernelGGL(k_control, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
 }
