@@ -182,6 +182,9 @@ int cubble_server_submit(cubble_server *s, const cubble_config *cfg, int64_t max
     if (!s || !cfg || max_steps <= 0) return CUBBLE_E_INVALID;
     if (cfg->precision != s->opts.precision)
         return s->e.fail(CUBBLE_E_INVALID, "request precision differs from the server's");
+    // every resident foam runs under one set of launch flags, without flow
+    if (cfg->flow)
+        return s->e.fail(CUBBLE_E_UNSUPPORTED, "the serving loop does not run the imposed flow");
     Request q;
     q.cfg = *cfg;
     q.max_steps = max_steps;

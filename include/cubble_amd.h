@@ -389,7 +389,9 @@ typedef struct cubble_server_stats {
 } cubble_server_stats;
 
 int cubble_server_create(const cubble_server_opts *opts, cubble_server **out);
-/* Queue a request arriving `arrival_s` seconds after the run starts. */
+/* Queue a request arriving `arrival_s` seconds after the run starts.  A
+ * request with imposed flow (cfg->flow) is refused with
+ * CUBBLE_E_UNSUPPORTED: the resident foams share one set of launch flags. */
 int cubble_server_submit(cubble_server *s, const cubble_config *cfg, int64_t max_steps,
                          double dissolve_fraction, double arrival_s, int64_t *id);
 /* Serve every queued request (arrivals in real time); returns when all are done. */

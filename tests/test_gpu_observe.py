@@ -309,3 +309,32 @@ def test_time_kernels_leaves_the_run_unchanged(precision):
         assert np.array_equal(a[k], b[k]), k
     sim.close()
     twin.close()
+
+
+def test_cli_runs_the_imposed_flow(tmp_path, monkeypatch, capfd):
+    """flow.impose = 1 in the input runs the loop with the flow: the mean
+    x velocity in results.dat follows the imposed flow's direction."""
+    src = os.path.join(os.path.dirname(__file__), "golden", "input_small_2d.json")
+    with open(src) as fh:
+        j = json.load(fh)
+    j["constants"]["K"]["value"] = 100.0
+    j["run"]["maxSteps"] = 1500
+    j["stabilization"] = {"steps": 0, "maxDeltaEnergy": 1.0}
+    means = {}
+    for impose in (0, 1):
+        d = tmp_path / ("flow%d" % impose)
+        d.mkdir()
+        j["flow"] = {"impose": impose, "velocity": {"x": 0.5, "y": 0.0, "z": 0.0},
+                     "lbb": {"x": 0.0, "y": 0.0, "z": 0.0}, "tfr": {"x": 1.0, "y": 0.5, "z": 0.0}}
+        path = str(d / "input.json")
+        with open(path, "w") as fh:
+            json.dump(j, fh)
+        monkeypatch.chdir(d)
+        print("cli flow", impose, flush=True)
+        assert ca.run_cli(["cubble-cli", path]) == 0
+        with open(d / "results.dat") as fh:
+            rows = [line.split(" ") for line in fh.read().splitlines()]
+        assert rows
+        means[impose] = float(rows[-1][6])   # mean dx/dt
+    capfd.readouterr()
+    assert means[1] > means[0] + 0.05

@@ -79,6 +79,14 @@ def test_server_rejects_request_larger_than_budget():
     srv.close()
 
 
+def test_server_refuses_flow_requests():
+    srv = ca.Server(hbm_budget_bytes=1 << 28, round_steps=10)
+    with pytest.raises(ca.CubbleError) as e:
+        srv.submit(config(2, 2000, seed=3, flow=1), max_steps=10)
+    assert e.value.code == ca.E_UNSUPPORTED
+    srv.close()
+
+
 def test_server_dissolution_stop():
     # a broad size distribution with strong gas exchange dissolves bubbles quickly
     cfg = config(2, 3000, dist=1, phi=0.9, seed=9, r_std=0.5, kappa=0.5, k_param=1.0)
