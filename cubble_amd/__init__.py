@@ -66,7 +66,7 @@ class _Config(ctypes.Structure):
         ("nbr_cap", ctypes.c_int32), ("cell_cap_factor", ctypes.c_double),
         ("max_steps", ctypes.c_int64), ("flow", ctypes.c_int32),
         ("flow_vel", ctypes.c_double * 3), ("flow_lbb", ctypes.c_double * 3),
-        ("flow_tfr", ctypes.c_double * 3),
+        ("flow_tfr", ctypes.c_double * 3), ("placement", ctypes.c_int32),
     ]
 
 
@@ -322,6 +322,7 @@ class Config:
     flow_vel: tuple = (0.01, 0.0, 0.0)
     flow_lbb: tuple = (0.0, 0.475, 0.0)
     flow_tfr: tuple = (0.4, 0.525, 0.0)
+    placement: int = 0            # 0 uniform, 1 the reference's lattice
 
     _VEC = ("rel_dim", "wall", "flow_vel", "flow_lbb", "flow_tfr")
 

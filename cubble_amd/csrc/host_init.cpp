@@ -286,6 +286,24 @@ void host_cell_dims(const Consts &c, double max_r, long long cap, int cd[3]) {
     }
 }
 
+void lattice_dims(const cubble_config &cfg, int32_t bpd[3]) {
+    float n = (float)cfg.num_start;
+    const float rx = (float)cfg.rel_dim[0], ry = (float)cfg.rel_dim[1], rz = (float)cfg.rel_dim[2];
+    if (3 == cfg.dim) {
+        n = std::cbrt(n);
+        const float a = std::cbrt(rx / ry), b = std::cbrt(rx / rz), c = std::cbrt(ry / rz);
+        bpd[0] = (int32_t)std::ceil(n * a * b);
+        bpd[1] = (int32_t)std::ceil(n * c / a);
+        bpd[2] = (int32_t)std::ceil(n / (b * c));
+    } else {
+        n = std::sqrt(n);
+        const float a = std::sqrt(rx / ry);
+        bpd[0] = (int32_t)std::ceil(n * a);
+        bpd[1] = (int32_t)std::ceil(n / a);
+        bpd[2] = 1;
+    }
+}
+
 int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &out,
                   std::string &err) {
     const int D = cfg.dim;
@@ -296,6 +314,19 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
     }
     if (n < 1) {
         err = "bubbles.numStart must be >= 1";
+        return CUBBLE_E_INVALID;
+    }
+    int32_t bpd[3] = {1, 1, 1};
+    if (1 == cfg.placement) {
+        lattice_dims(cfg, bpd);
+        const long long lat = (long long)bpd[0] * bpd[1] * bpd[2];
+        if (lat != n) {
+            err = "init.placement \"grid\": num_start must be the lattice count " +
+                  std::to_string(lat);
+            return CUBBLE_E_INVALID;
+        }
+    } else if (0 != cfg.placement) {
+        err = "init.placement must be \"uniform\" or \"grid\"";
         return CUBBLE_E_INVALID;
     }
     if (!(cfg.phi > 0.0 && cfg.phi < 1.0) || !(cfg.r_mean > 0.0) || !(cfg.mu > 0.0)) {
@@ -363,8 +394,33 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
         maxr = std::fmax(maxr, r[i]);
     }
     c.avg_area_in = ain;
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < D; ++k) x4[(size_t)k * n + i] = c.lbb[k] + c.interval[k] * rng.uniform();
+    if (1 == cfg.placement) {
+        // the reference's lattice (assignDataToBubbles, src/kernels.cu:1034-1056):
+        // site i at ((i mod bx) / bx, ...) of the box, plus an offset of length
+        // R w along the normalised vector of D uniforms (so in the positive
+        // orthant), folded back into the box; then, as the reference's rescale
+        // (src/simulator.cu:797-817), the relative positions are kept in the
+        // box sized for the target gas fraction
+        for (int i = 0; i < n; ++i) {
+            const double site[3] = {(double)(i % bpd[0]) / bpd[0],
+                                    (double)((i / bpd[0]) % bpd[1]) / bpd[1],
+                                    (double)(i / (bpd[0] * bpd[1])) / bpd[2]};
+            double u[3] = {rng.uniform(), rng.uniform(), 3 == D ? rng.uniform() : 0.0};
+            const double w = rng.uniform();
+            const double len = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            for (int k = 0; k < D; ++k) {
+                // lattice box: edge 2 R per site
+                const double lat = 2.0 * cfg.r_mean * bpd[k];
+                double p = site[k] * lat + (len > 0.0 ? u[k] / len : 0.0) * cfg.r_mean * w;
+                p = p > 0.0 ? (p < lat ? p : p - lat) : p + lat;
+                x4[(size_t)k * n + i] = c.lbb[k] + c.interval[k] * (p / lat);
+            }
+        }
+    } else {
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < D; ++k)
+                x4[(size_t)k * n + i] = c.lbb[k] + c.interval[k] * rng.uniform();
+    }
     out.max_r = maxr;
     int cd[3];
     host_cell_dims(c, maxr, 1LL << 30, cd);
@@ -440,6 +496,14 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
     if (jnum(root, {"init", "neighbourCapacity"}, v)) cfg.nbr_cap = (int32_t)v;
     if (jnum(root, {"init", "cellCapacityFactor"}, v)) cfg.cell_cap_factor = v;
     if (jnum(root, {"run", "maxSteps"}, v)) cfg.max_steps = (int64_t)v;
+    if (jstr(root, {"init", "placement"}, s)) {
+        if (s == "grid") cfg.placement = 1;
+        else if (s == "uniform") cfg.placement = 0;
+        else {
+            err = "init.placement must be \"uniform\" or \"grid\"";
+            return CUBBLE_E_INVALID;
+        }
+    }
     if (jstr(root, {"init", "precision"}, s)) {
         if (s == "mixed") cfg.precision = CUBBLE_PREC_MIXED;
         else if (s == "fp64") cfg.precision = CUBBLE_PREC_FP64;
@@ -447,6 +511,13 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
             err = "init.precision must be \"fp64\" or \"mixed\"";
             return CUBBLE_E_INVALID;
         }
+    }
+    // the reference places numStart rounded up to a full lattice
+    // (src/simulator.cu:576-600)
+    if (1 == cfg.placement) {
+        int32_t bpd[3];
+        lattice_dims(cfg, bpd);
+        cfg.num_start = bpd[0] * bpd[1] * bpd[2];
     }
     return CUBBLE_OK;
 }

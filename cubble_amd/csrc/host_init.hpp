@@ -24,6 +24,11 @@ void host_cell_dims(const Consts &c, double max_r, long long cap, int cd[3]);
 int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &out,
                   std::string &err);
 
+// Bubbles per axis of the reference's initial lattice for `num_start`
+// bubbles (src/simulator.cu:576-600, in its single-precision arithmetic); the
+// lattice count is their product.
+void lattice_dims(const cubble_config &cfg, int32_t bpd[3]);
+
 // input_parameters.json (reference schema) -> config.
 int parse_config_json(const char *path, cubble_config &cfg, std::string &err);
 

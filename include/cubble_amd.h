@@ -103,6 +103,13 @@ typedef struct cubble_config {
     double flow_vel[3];       /* flow.velocity, in units of f0 = sigma R / mu  */
     double flow_lbb[3];       /* flow.lbb, relative to the box (0..1)          */
     double flow_tfr[3];       /* flow.tfr, relative to the box (0..1)          */
+    int32_t placement;        /* 0 = uniform positions in the box; 1 = the
+                                 reference's lattice (spacing 2 R, a random
+                                 offset of up to R per bubble, then scaled to
+                                 the box; src/simulator.cu:576-606,
+                                 src/kernels.cu:1026-1064).  With 1, num_start
+                                 must be the lattice count (the JSON reader
+                                 rounds numStart up to it, as the reference)   */
 } cubble_config;
 
 /* State view: caller-owned host buffers, filled by cubble_get_state or read

@@ -95,3 +95,52 @@ def test_invalid_config_rejected_on_host():
 def test_cli_usage_error_without_device_work():
     # wrong argument count: returns EXIT_FAILURE before touching the GPU
     assert ca.run_cli(["cubble-cli"]) == 1
+
+
+# --------------------------------------------------------------------------
+# the reference's lattice placement (init.placement "grid")
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,n,rel,want", [
+    (2, 2000, (1.0, 1.0, 1.0), 45 * 45), (2, 2025, (1.0, 1.0, 1.0), 45 * 45),
+    (2, 2026, (1.0, 1.0, 1.0), 46 * 46), (2, 2000, (2.0, 1.0, 1.0), 64 * 32),
+    (3, 1000, (1.0, 1.0, 1.0), 10 ** 3), (3, 1001, (1.0, 1.0, 1.0), 11 ** 3),
+    (3, 5000, (1.0, 1.0, 1.0), 18 ** 3)])
+def test_grid_placement_rounds_to_the_lattice(tmp_path, dim, n, rel, want):
+    # numStart rounds up to a full lattice (src/simulator.cu:576-600); a
+    # programmatic config must already hold the lattice count
+    j = {"bubbles": {"numStart": n, "numEnd": 10, "radius": {"mean": 1.0, "std": 0.1}},
+         "box": {"dimensionality": dim,
+                 "relativeDimensions": {"x": rel[0], "y": rel[1], "z": rel[2]}},
+         "init": {"placement": "grid"}}
+    p = tmp_path / "in.json"
+    p.write_text(__import__("json").dumps(j))
+    c = ca.Config.from_json(str(p))
+    assert c.placement == 1 and c.num_start == want
+    if want != n:
+        with pytest.raises(ca.CubbleError) as e:
+            ca.generate_foam(ca.Config(dim=dim, num_start=n, placement=1, rel_dim=rel))
+        assert e.value.code == ca.E_INVALID and str(want) in str(e.value)
+
+
+@pytest.mark.parametrize("dim,bpd", [(2, (45, 45, 1)), (3, (12, 12, 12))])
+def test_grid_placement_positions(dim, bpd):
+    """Every bubble sits within R (scaled to the box) of its lattice site, in
+    the positive orthant of it (modulo the box), and the box holds phi
+    exactly."""
+    n = bpd[0] * bpd[1] * bpd[2]
+    cfg = ca.Config(dim=dim, num_start=n, placement=1, phi=0.9, r_mean=1.0, r_std=0.1)
+    x4, sc = ca.generate_foam(cfg)
+    L = np.array(sc["interval"][:dim])
+    i = np.arange(n)
+    site = np.stack([(i % bpd[0]) / bpd[0], ((i // bpd[0]) % bpd[1]) / bpd[1],
+                     (i // (bpd[0] * bpd[1])) / bpd[2]])[:dim] * L[:, None]
+    d = (x4[:dim] - site) % L[:, None]                   # offset, folded into the box
+    scale = L / (2.0 * np.array(bpd[:dim]))              # R in box units, per axis
+    assert np.all(d / scale[:, None] <= 1.0 + 1e-9)
+    assert np.all(np.sqrt(((d / scale[:, None]) ** 2).sum(0)) <= 1.0 + 1e-9)
+    assert np.all((x4[:dim] >= 0) & (x4[:dim] <= L[:, None]))
+    r = x4[3]
+    vol = np.pi * r ** 2 * (4.0 / 3.0 * r if dim == 3 else 1.0)
+    assert vol.sum() == pytest.approx(0.9 * np.prod(L), rel=1e-12)
+    # not all offsets vanish
+    assert np.std(d) > 0.1 * scale.min()

@@ -303,3 +303,27 @@ def test_json_create_and_cli_run(tmp_path):
     assert sim.scalars()["count"] == 2000
     sim.close()
     assert ca.run_cli(["cubble-cli", path]) == 0
+
+
+@pytest.mark.parametrize("dim,n,seed", [(2, 2025, 61), (3, 4096, 62)])
+def test_grid_placement_stepwise_from_shared_state(ora, dim, n, seed):
+    """A foam started on the reference's lattice (init.placement "grid"):
+    the first search and 30 steps, each started on both sides from the GPU's
+    state."""
+    print(f"grid placement dim={dim} n={n} seed={seed}", flush=True)
+    sim = ca.Simulator(config(dim, n, seed=seed, placement=1))
+    f = oracle_from_sim(ora, sim)
+    f.search()
+    assert_same_structure(sim.get_state()[0], f.get_state()[0])
+    for step in range(30):
+        f = oracle_from_sim(ora, sim)
+        sim.integrate(GAS)
+        f.integrate(True, True)
+        gst, gsc = sim.get_state()
+        ost, osc = f.get_state()
+        assert gsc["count"] == osc["count"] and gsc["searches"] == osc["searches"]
+        assert_same_structure(gst, ost)
+        f0 = gsc["f0"]
+        assert max(scaled_err(gst["v"][k], ost["v"][k], f0) for k in range(dim)) <= FP64_TOL
+        assert np.allclose(gst["x"], ost["x"], rtol=1e-13, atol=1e-13)
+    sim.close()
