@@ -178,7 +178,7 @@ EXPORTS = [
     "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
     "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
     "cubble_batch_retune", "cubble_observe", "cubble_set_stop_rules",
-    "cubble_reset_print_window", "cubble_reset_clock",
+    "cubble_reset_print_window", "cubble_reset_clock", "cubble_bootstrap_velocities",
 ]
 
 
@@ -241,6 +241,7 @@ def lib():
         L.cubble_set_stop_rules.argtypes = [vp, P(_StopRules)]
         L.cubble_reset_print_window.argtypes = [vp]
         L.cubble_reset_clock.argtypes = [vp, ctypes.c_int32]
+        L.cubble_bootstrap_velocities.argtypes = [vp]
         L.cubble_create_tuned.argtypes = [P(_Config), P(_Tuning), P(vp)]
         L.cubble_batch_create.argtypes = [P(_Config), ctypes.c_int32, P(_Tuning), P(vp)]
         L.cubble_batch_size.argtypes = [vp]
@@ -522,6 +523,10 @@ class Simulator:
     def reset_print_window(self):
         """Zero the per-bubble error and restart the time-step window."""
         self._call(lib().cubble_reset_print_window(self._h))
+
+    def bootstrap_velocities(self):
+        """The AB2 velocity bootstrap of the reference's init (new foam only)."""
+        self._call(lib().cubble_bootstrap_velocities(self._h))
 
     def reset_clock(self, clear_wraps=False):
         """Time to zero; with clear_wraps also the wrap counters."""

@@ -1,7 +1,7 @@
 // cli.cpp -- drop-in command line entry, cubble_run(argc, argv), and the
 // one-line main of cubble-cli.  Mirrors cubble::run (reference
-// src/simulator.cu:1161-1186), the stabilisation phase of init()
-// (src/simulator.cu:789-791, :848-955) and simulate()
+// src/simulator.cu:1161-1186), the velocity bootstrap and stabilisation
+// phase of init() (src/simulator.cu:739-791, :848-955) and simulate()
 // (src/simulator.cu:958-1157): argument check, device check, stabilisation
 // with its table, the integration loop and its stop rule, the printed table,
 // results.dat and the snapshot CSVs.  The foam is the synthetic one drawn at
@@ -236,6 +236,9 @@ extern "C" int cubble_run(int argc, char **argv) {
         0.5 * (3 == sc.dim ? std::fmin(L[0], std::fmin(L[1], L[2])) : std::fmin(L[0], L[1]));
     // timeScalingFactor = K / avgRad^2 (src/simulator.cu:557-559)
     const double time_scale = cfg.k_param / (cfg.r_mean * cfg.r_mean);
+    // the AB2 history of init (src/simulator.cu:739-787)
+    std::printf("Calculating initial velocities for Adams-Bashforth-Moulton\n");
+    if (cubble_bootstrap_velocities(sim)) return fail(sim);
     int64_t row_searches0 = sc.searches;
     if (rs.stabilization_steps > 0 &&
         stabilization_phase(sim, rs.stabilization_steps, rs.max_delta_energy, time_scale,

@@ -47,6 +47,11 @@ void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear
 // add > 0: every foam halts after `add` more accepted steps; 0: no limit.
 void launch_set_limits(const Grid &g, long long add);
 
+// The AB2 velocity bootstrap of init: pair velocities (no gas, flow or
+// walls) at the positions moved by one Euler step of the pair velocities
+// become the old velocities.  Needs zero velocities (a new foam).
+void launch_bootstrap(const Grid &g, bool mixed);
+
 // One foam's observables (h: a current host copy of its header, dev: its
 // device header): out[0..7] = total energy, total volume, then the sums of
 // r, path, dxdt, dydt, dzdt, drdt; energy[i] (if not null) = bubble i's

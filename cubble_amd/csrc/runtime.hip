@@ -392,6 +392,19 @@ int cubble_reset_print_window(cubble_sim *s) {
     return e.write_dyn(0, h.d);
 }
 
+int cubble_bootstrap_velocities(cubble_sim *s) {
+    if (!s) return CUBBLE_E_INVALID;
+    Engine &e = s->e;
+    Foam h;
+    int rc = e.read_header(0, h);
+    if (rc) return rc;
+    if (h.d.attempts != 0 || h.d.steps != 0)
+        return e.fail(CUBBLE_E_STATE, "bootstrap_velocities: only before the first attempt");
+    launch_bootstrap(e.grid(), CUBBLE_PREC_MIXED == e.precision);
+    R_TRY(e, hipGetLastError());
+    return e.sync();
+}
+
 int cubble_reset_clock(cubble_sim *s, int32_t clear_wraps) {
     if (!s) return CUBBLE_E_INVALID;
     Engine &e = s->e;

@@ -508,6 +508,71 @@ void launch_sums(const Grid &g) {
     hipLaunchKernelGGL(k_sums, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams);
 }
 
+// AB2 bootstrap of init (reference src/simulator.cu:739-787): the Euler step
+// of the predicted record by the pair velocities in vp (kernel euler,
+// src/kernels.cu:958-968), then those velocities become the old ones.  The
+// record holds the current positions (zero velocities at creation); in mixed
+// precision the stepped FP64 position is recoded into the fixed-point record.
+__global__ __launch_bounds__(kBlock) void k_euler(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    const int n = F->d.count, ipt = F->ipt, D = F->c.dim;
+    const Consts c = F->c;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const double ts = F->d.dt;
+    const int base = cx.lb * kBlock * ipt;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        if (S.q4) {
+            Q4 q = S.q4[i];
+            auto code = [&](int a) {
+                const double x = R.x[a][i] + (double)field<float>(R.vp[a])[i] * ts;
+                return fixed_code(x, c.lbb[a], c.interval[a], c.wall[a] != 0);
+            };
+            q.x = code(0);
+            q.y = code(1);
+            if (3 == D) q.z = code(2);
+            S.q4[i] = q;
+        } else {
+            D4 p = S.p4[i];
+            p.x += field<double>(R.vp[0])[i] * ts;
+            p.y += field<double>(R.vp[1])[i] * ts;
+            if (3 == D) p.z += field<double>(R.vp[2])[i] * ts;
+            S.p4[i] = p;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_old_from_pair(Foam *foams, const int2 *bt) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    const int n = F->d.count, ipt = F->ipt, D = F->c.dim;
+    const Roles R = F->r;
+    const bool mixed = F->s.q4 != nullptr;
+    const int base = cx.lb * kBlock * ipt;
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (i >= n) break;
+        for (int a = 0; a < D; ++a) {
+            if (mixed) field<float>(R.vo[a])[i] = field<float>(R.vp[a])[i];
+            else field<double>(R.vo[a])[i] = field<double>(R.vp[a])[i];
+        }
+    }
+}
+
+void launch_bootstrap(const Grid &g, bool mixed) {
+    launch_predict(g, 0);  // zero velocities: the record holds the positions
+    if (mixed) launch_pair_mixed(g, 0);
+    else launch_pair(g, 0);
+    hipLaunchKernelGGL(k_euler, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
+    if (mixed) launch_pair_mixed(g, 0);
+    else launch_pair(g, 0);
+    hipLaunchKernelGGL(k_old_from_pair, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
+                       g.bt_bub);
+}
+
 void launch_correct(const Grid &g, uint32_t flags, bool mixed, bool control) {
     if (mixed)
         hipLaunchKernelGGL(k_correct<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,

@@ -266,6 +266,13 @@ int cubble_set_stop_rules(cubble_sim *sim, const cubble_stop_rules *rules);
  * cubble_scalars.ts_min / ts_max / ts_sum report. */
 int cubble_reset_print_window(cubble_sim *sim);
 
+/* The AB2 velocity bootstrap of the reference's init (src/simulator.cu:
+ * 739-787): the pair velocities (no gas exchange, flow or walls) at the
+ * positions moved by one Euler step of dt along the pair velocities become
+ * the old velocities; the current ones stay zero.  Only before the first
+ * attempt (CUBBLE_E_STATE otherwise); without it the history starts at zero. */
+int cubble_bootstrap_velocities(cubble_sim *sim);
+
 /* What the reference's init resets before each stabilisation round and
  * before the integration loop (src/simulator.cu:866-867, :938-955): the
  * clock to zero and, with clear_wraps, the wrap counters, so that unwrapped

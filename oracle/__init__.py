@@ -158,6 +158,7 @@ def lib():
         L.ora_init_synthetic.argtypes = [ctypes.POINTER(InitParams)]
         L.ora_init_synthetic.restype = vp
         L.ora_observe.argtypes = [vp, _dp, _dp]
+        L.ora_bootstrap.argtypes = [vp]
         L.ora_cbrt.argtypes = [ctypes.c_double]
         L.ora_cbrt.restype = ctypes.c_double
         u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -357,6 +358,9 @@ class Foam:
 
     def run(self, steps, gas=True, path=True, flow=False):
         return lib().ora_run(self.h, int(gas), int(path), int(flow), int(steps))
+
+    def bootstrap(self):
+        lib().ora_bootstrap(self.h)
 
     def search(self):
         lib().ora_search(self.h)

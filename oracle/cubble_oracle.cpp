@@ -177,36 +177,13 @@ void resize_attempt(Foam &f) {
     f.del.assign(n, 0);
 }
 
-/* One step() attempt: reference src/simulator.cu:192-249 with kernels
- * preIntegrate (src/kernels.cu:40-170), pairwiseInteraction (:172-337) and
- * postIntegrate (:339-495). */
-int attempt(Foam &f, bool gas, bool incr_path, bool flow) {
+/* pairwise velocity and gas flux of the predicted state f.xp into f.vp and
+ * f.ov, gathered per bubble: src/kernels.cu:172-337 (pair math :210-280,
+ * flow :191-208, :281-307), :1094-1114 (wrap), vec.h:41-49 */
+void pair_step(Foam &f, bool gas, bool flow, double &s_oa, double &s_oar) {
     const ora_consts &c = f.c;
     const int32_t n = f.n;
     const int D = c.dim;
-    const double ts = f.s.dt;
-    resize_attempt(f);
-
-    /* predictor + S_A, S_Ar: src/kernels.cu:48-113 */
-    double s_a = 0.0, s_ar = 0.0;
-#pragma omp parallel for reduction(+ : s_a, s_ar) if (g_threads > 1) num_threads(g_threads) schedule(static)
-    for (int32_t i = 0; i < n; ++i) {
-        for (int k = 0; k < D; ++k) f.xp[k][i] = predict(f.x[k][i], f.v[k][i], f.vo[k][i], ts);
-        if (gas) {
-            const double rp = predict(f.x[3][i], f.v[3][i], f.vo[3][i], ts);
-            f.xp[3][i] = rp;
-            double apr = 2.0 * kPi;
-            if (3 == D) apr *= 2.0 * rp;
-            s_a += apr * rp;
-            s_ar += apr;
-        } else {
-            f.xp[3][i] = f.x[3][i];
-        }
-    }
-
-    /* pairwise velocity and gas flux, gathered per bubble:
-     * src/kernels.cu:210-280 (pair math), :1094-1114 (wrap), vec.h:41-49 */
-    double s_oa = 0.0, s_oar = 0.0;
 #pragma omp parallel for reduction(+ : s_oa, s_oar) if (g_threads > 1) num_threads(g_threads) schedule(static)
     for (int32_t i = 0; i < n; ++i) {
         double acc[3] = {0.0, 0.0, 0.0};
@@ -264,6 +241,37 @@ int attempt(Foam &f, bool gas, bool incr_path, bool flow) {
         s_oa += ovl;
         s_oar += ovl * (1.0 / ri);
     }
+}
+
+/* One step() attempt: reference src/simulator.cu:192-249 with kernels
+ * preIntegrate (src/kernels.cu:40-170), pairwiseInteraction (:172-337) and
+ * postIntegrate (:339-495). */
+int attempt(Foam &f, bool gas, bool incr_path, bool flow) {
+    const ora_consts &c = f.c;
+    const int32_t n = f.n;
+    const int D = c.dim;
+    const double ts = f.s.dt;
+    resize_attempt(f);
+
+    /* predictor + S_A, S_Ar: src/kernels.cu:48-113 */
+    double s_a = 0.0, s_ar = 0.0;
+#pragma omp parallel for reduction(+ : s_a, s_ar) if (g_threads > 1) num_threads(g_threads) schedule(static)
+    for (int32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < D; ++k) f.xp[k][i] = predict(f.x[k][i], f.v[k][i], f.vo[k][i], ts);
+        if (gas) {
+            const double rp = predict(f.x[3][i], f.v[3][i], f.vo[3][i], ts);
+            f.xp[3][i] = rp;
+            double apr = 2.0 * kPi;
+            if (3 == D) apr *= 2.0 * rp;
+            s_a += apr * rp;
+            s_ar += apr;
+        } else {
+            f.xp[3][i] = f.x[3][i];
+        }
+    }
+
+    double s_oa = 0.0, s_oar = 0.0;
+    pair_step(f, gas, flow, s_oa, s_oar);
 
     /* walls, liquid-mediated gas exchange, corrector, error, expansion,
      * deletion marks, volume, path: src/kernels.cu:356-456, :487-495 */
@@ -938,6 +946,26 @@ void ora_observe(void *p, double out[8], double *energy) {
         for (int q = 0; q < 4; ++q)
             if (2 != q || 3 == c.dim) out[4 + q] += f.v[q][i];
     }
+}
+
+/* The AB2 velocity bootstrap of init (reference src/simulator.cu:739-787,
+ * kernel euler src/kernels.cu:958-968): with the predicted state equal to
+ * the current one, the pair velocities v0 (no gas exchange, no flow, no
+ * walls); an Euler step x1 = x0 + v0 dt of the predicted positions; the
+ * pair velocities at x1 become the old velocities.  The current velocities
+ * are left as they are (zero at creation). */
+void ora_bootstrap(void *p) {
+    Foam &f = *static_cast<Foam *>(p);
+    const int D = f.c.dim;
+    const double ts = f.s.dt;
+    resize_attempt(f);
+    for (int k = 0; k < 4; ++k) f.xp[k] = f.x[k];
+    double s_oa = 0.0, s_oar = 0.0;
+    pair_step(f, false, false, s_oa, s_oar);
+    for (int k = 0; k < D; ++k)
+        for (int32_t i = 0; i < f.n; ++i) f.xp[k][i] += f.vp[k][i] * ts;
+    pair_step(f, false, false, s_oa, s_oar);
+    for (int k = 0; k < D; ++k) f.vo[k] = f.vp[k];
 }
 
 double ora_cbrt(double a) { return det_cbrt(a); }

@@ -127,6 +127,10 @@ void *ora_init_synthetic(const ora_init_params *p);
  * out[0..7] = total energy, total volume, then the sums of r, path, dxdt,
  * dydt, dzdt, drdt; energy (count doubles or NULL) = each bubble's energy. */
 void ora_observe(void *f, double out[8], double *energy);
+
+/* The AB2 velocity bootstrap of init: old velocities = pair velocities
+ * after one Euler step of the pair velocities (no gas, flow or walls). */
+void ora_bootstrap(void *f);
 /* Deterministic cube root used by the volume fix. */
 double ora_cbrt(double a);
 /* Raw outputs of the project RNG, for known-answer tests: SplitMix64 from

@@ -158,6 +158,7 @@ def _replay(json_path, rows):
     tsf = j["constants"]["K"]["value"] / (r_mean * r_mean)
     steps = int(j["stabilization"]["steps"])
     sim = ca.Simulator(json_path=json_path)
+    sim.bootstrap_velocities()
     for _ in range(5):
         _stabilize(sim, steps)
     searches0 = sim.scalars()["searches"]

@@ -327,3 +327,37 @@ def test_grid_placement_stepwise_from_shared_state(ora, dim, n, seed):
         assert max(scaled_err(gst["v"][k], ost["v"][k], f0) for k in range(dim)) <= FP64_TOL
         assert np.allclose(gst["x"], ost["x"], rtol=1e-13, atol=1e-13)
     sim.close()
+
+
+@pytest.mark.parametrize("dim,n,seed,precision", [(2, 3000, 71, ca.PREC_FP64),
+                                                  (3, 4000, 72, ca.PREC_FP64),
+                                                  (3, 6000, 73, ca.PREC_MIXED)])
+def test_velocity_bootstrap_matches_oracle(ora, dim, n, seed, precision):
+    """The AB2 bootstrap of init on a new foam, then (FP64; a mixed run may
+    accept a different attempt than the oracle) 10 steps from the shared
+    state; refused once a step has run."""
+    print(f"bootstrap dim={dim} n={n} seed={seed} precision={precision}", flush=True)
+    sim = ca.Simulator(config(dim, n, seed=seed, precision=precision))
+    f = oracle_from_sim(ora, sim)
+    sim.bootstrap_velocities()
+    f.bootstrap()
+    gst, gsc = sim.get_state()
+    ost, _ = f.get_state()
+    tol = FP64_TOL if precision == ca.PREC_FP64 else MIXED_TOL
+    f0 = gsc["f0"]
+    assert max(scaled_err(gst["vo"][k], ost["vo"][k], f0) for k in range(dim)) <= tol
+    assert np.max(np.abs(ost["vo"][:dim])) > 1e-3 * f0
+    assert not np.any(gst["v"]) and np.array_equal(gst["x"], ost["x"])
+    for _ in range(10 if precision == ca.PREC_FP64 else 0):
+        f = oracle_from_sim(ora, sim)
+        sim.integrate(GAS)
+        f.integrate(True, True)
+        gst, gsc = sim.get_state()
+        ost, _ = f.get_state()
+        assert_same_structure(gst, ost)
+        assert max(scaled_err(gst["v"][k], ost["v"][k], f0) for k in range(dim)) <= tol
+    sim.integrate(GAS)
+    with pytest.raises(ca.CubbleError) as e:
+        sim.bootstrap_velocities()
+    assert e.value.code == ca.E_STATE
+    sim.close()

@@ -589,3 +589,26 @@ def test_flow_box_edges_count_within_radius(ora, dim):
         vp = f.attempt_out()["vp"]
         want = vel[:dim] if inside else (0.0,) * dim
         assert tuple(vp[:dim, 0]) == want, (x, inside)
+
+
+# --------------------------------------------------------------------------
+# the AB2 velocity bootstrap of init
+# --------------------------------------------------------------------------
+def test_bootstrap_two_bubble_closed_form(ora):
+    """Two bubbles of radius 0.5 0.8 apart (f0 = 1): the pair velocities are
+    -/+0.2 along x; after an Euler step of dt they are 0.8 + 0.4 dt apart and
+    the velocities there, -/+(0.2 - 0.4 dt), become the old velocities.  A
+    third bubble far away keeps zero; the current velocities stay zero."""
+    dt = 1e-4
+    st = make_state([(10.0, 10.0), (10.8, 10.0), (3.0, 3.0)], [0.5, 0.5, 0.5], 2)
+    st["nbr_off"] = np.array([0, 1, 2, 2], np.int32)
+    st["nbr"] = np.array([1, 0], np.int32)
+    f = ora.Foam.from_state(consts(2), st, scalars(3, dt=dt))
+    f.bootstrap()
+    got, _ = f.get_state()
+    want = 0.2 - 0.4 * dt
+    assert got["vo"][0, 0] == pytest.approx(-want, rel=1e-12)
+    assert got["vo"][0, 1] == pytest.approx(want, rel=1e-12)
+    assert got["vo"][0, 2] == 0.0 and not np.any(got["vo"][1])
+    assert not np.any(got["v"])
+    assert np.array_equal(got["x"], st["x"])
