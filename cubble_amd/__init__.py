@@ -179,6 +179,7 @@ EXPORTS = [
     "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
     "cubble_batch_retune", "cubble_observe", "cubble_set_stop_rules",
     "cubble_reset_print_window", "cubble_reset_clock", "cubble_bootstrap_velocities",
+    "cubble_rescale_box",
 ]
 
 
@@ -242,6 +243,7 @@ def lib():
         L.cubble_reset_print_window.argtypes = [vp]
         L.cubble_reset_clock.argtypes = [vp, ctypes.c_int32]
         L.cubble_bootstrap_velocities.argtypes = [vp]
+        L.cubble_rescale_box.argtypes = [vp, ctypes.c_double, _dp]
         L.cubble_create_tuned.argtypes = [P(_Config), P(_Tuning), P(vp)]
         L.cubble_batch_create.argtypes = [P(_Config), ctypes.c_int32, P(_Tuning), P(vp)]
         L.cubble_batch_size.argtypes = [vp]
@@ -523,6 +525,11 @@ class Simulator:
     def reset_print_window(self):
         """Zero the per-bubble error and restart the time-step window."""
         self._call(lib().cubble_reset_print_window(self._h))
+
+    def rescale_box(self, phi, rel_dim=(1.0, 1.0, 1.0)):
+        """The box rescale of the reference's init (then a search)."""
+        rel = np.ascontiguousarray(rel_dim, dtype=np.float64)
+        self._call(lib().cubble_rescale_box(self._h, float(phi), _dptr(rel)))
 
     def bootstrap_velocities(self):
         """The AB2 velocity bootstrap of the reference's init (new foam only)."""

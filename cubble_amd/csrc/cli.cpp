@@ -72,19 +72,42 @@ int stabilize(cubble_sim *sim, int64_t steps, double &e1, double &e2) {
     return CUBBLE_OK;
 }
 
-// The stabilisation phase of init() (src/simulator.cu:789-791, :848-899):
-// five rounds, then rounds until the energy change per scaled time is below
-// maxDeltaEnergy (or small for a low energy), at most about 500, printing
-// the reference's table.  `searches0` returns the search count the
-// reference's counter starts the first integration row from (it is not
-// reset after the last round).
+// Five stabilisation rounds (src/simulator.cu:789-791, :848-850).
+int five_rounds(cubble_sim *sim, int64_t steps, const char *when) {
+    std::printf("Stabilizing a few rounds after %s\n", when);
+    double e1 = 0.0, e2 = 0.0;
+    for (int k = 0; k < 5; ++k) {
+        const int rc = stabilize(sim, steps, e1, e2);
+        if (rc) return rc;
+    }
+    return CUBBLE_OK;
+}
+
+// The box rescale of init (src/simulator.cu:793-846) for a foam placed on
+// the lattice in its own box.
+int scale_box(cubble_sim *sim, const cubble_config &cfg) {
+    std::printf("Scaling the simulation box\n");
+    cubble_observables o;
+    int rc = cubble_observe(sim, &o, nullptr);
+    if (rc) return rc;
+    std::printf("Current phi: %.9g, target phi: %.9g\n", o.total_volume / o.box_volume, cfg.phi);
+    if ((rc = cubble_rescale_box(sim, cfg.phi, cfg.rel_dim)) ||
+        (rc = cubble_observe(sim, &o, nullptr)))
+        return rc;
+    std::printf("Current phi: %.9g, target phi: %.9g\n", o.total_volume / o.box_volume, cfg.phi);
+    std::printf("Neighbor search after scaling\n");
+    return CUBBLE_OK;
+}
+
+// The stabilisation loop of init() (src/simulator.cu:852-899): rounds until
+// the energy change per scaled time is below maxDeltaEnergy (or small for a
+// low energy), at most about 500, printing the reference's table.
+// `searches0` returns the search count the reference's counter starts the
+// first integration row from (it is not reset after the last round).
 int stabilization_phase(cubble_sim *sim, int64_t steps, double max_de, double time_scale,
                         int64_t &searches0) {
     double e1 = 0.0, e2 = 0.0;
     int rc = CUBBLE_OK;
-    std::printf("Stabilizing a few rounds after creation\n");
-    for (int k = 0; k < 5; ++k)
-        if ((rc = stabilize(sim, steps, e1, e2))) return rc;
     std::printf("\n=============\nStabilization\n=============\n");
     std::printf("%-7s %-11s %-11s %-11s %-9s\n", "#steps", "dE", "e1", "e2", "#searches");
     cubble_scalars sc;
@@ -225,24 +248,31 @@ extern "C" int cubble_run(int argc, char **argv) {
         return EXIT_FAILURE;
     }
     const double snap_freq = rs.snapshot_frequency;
+    // a lattice start is created in the lattice's own box and scaled to the
+    // target gas fraction after the first stabilisation rounds, as init does
+    const bool lattice = 1 == cfg.placement;
+    cubble_config created = cfg;
+    if (lattice) created.placement = 2;
     cubble_sim *sim = nullptr;
-    if (cubble_create(&cfg, &sim) != CUBBLE_OK) return fail(nullptr);
+    if (cubble_create(&created, &sim) != CUBBLE_OK) return fail(nullptr);
     int rc = CUBBLE_OK;
     cubble_scalars sc;
     cubble_observables obs;
     if ((rc = cubble_get_scalars(sim, &sc)) != CUBBLE_OK) return fail(sim);
-    const double *L = sc.interval;
-    const double min_interval =
-        0.5 * (3 == sc.dim ? std::fmin(L[0], std::fmin(L[1], L[2])) : std::fmin(L[0], L[1]));
+    const double *L = sc.interval;  // refreshed below, after any rescale
     // timeScalingFactor = K / avgRad^2 (src/simulator.cu:557-559)
     const double time_scale = cfg.k_param / (cfg.r_mean * cfg.r_mean);
     // the AB2 history of init (src/simulator.cu:739-787)
     std::printf("Calculating initial velocities for Adams-Bashforth-Moulton\n");
     if (cubble_bootstrap_velocities(sim)) return fail(sim);
+    const int64_t ssteps = rs.stabilization_steps;
+    if (ssteps > 0 && five_rounds(sim, ssteps, "creation")) return fail(sim);
+    if (lattice && (scale_box(sim, cfg) || (ssteps > 0 && five_rounds(sim, ssteps, "scaling"))))
+        return fail(sim);
+    if (cubble_get_scalars(sim, &sc)) return fail(sim);
     int64_t row_searches0 = sc.searches;
-    if (rs.stabilization_steps > 0 &&
-        stabilization_phase(sim, rs.stabilization_steps, rs.max_delta_energy, time_scale,
-                            row_searches0))
+    if (ssteps > 0 &&
+        stabilization_phase(sim, ssteps, rs.max_delta_energy, time_scale, row_searches0))
         return fail(sim);
     // integration starts from zero time, wrap counters and errors, with the
     // energy of the stabilised foam (src/simulator.cu:938-955)
@@ -251,6 +281,8 @@ extern "C" int cubble_run(int argc, char **argv) {
         return fail(sim);
     double e1 = obs.total_energy;
     const int64_t steps0 = sc.steps;  // run.maxSteps counts integration steps
+    const double min_interval =
+        0.5 * (3 == sc.dim ? std::fmin(L[0], std::fmin(L[1], L[2])) : std::fmin(L[0], L[1]));
 
     Snapshots snaps(rs.snapshot_name);
     int64_t num_snaps = 0;

@@ -317,7 +317,7 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
         return CUBBLE_E_INVALID;
     }
     int32_t bpd[3] = {1, 1, 1};
-    if (1 == cfg.placement) {
+    if (1 == cfg.placement || 2 == cfg.placement) {
         lattice_dims(cfg, bpd);
         const long long lat = (long long)bpd[0] * bpd[1] * bpd[2];
         if (lat != n) {
@@ -326,7 +326,7 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
             return CUBBLE_E_INVALID;
         }
     } else if (0 != cfg.placement) {
-        err = "init.placement must be \"uniform\" or \"grid\"";
+        err = "placement must be 0 (uniform), 1 (grid) or 2 (grid in the lattice box)";
         return CUBBLE_E_INVALID;
     }
     if (!(cfg.phi > 0.0 && cfg.phi < 1.0) || !(cfg.r_mean > 0.0) || !(cfg.mu > 0.0)) {
@@ -394,7 +394,21 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
         maxr = std::fmax(maxr, r[i]);
     }
     c.avg_area_in = ain;
-    if (1 == cfg.placement) {
+    if (2 == cfg.placement) {
+        // the lattice's own box, edge (float)(2 R) per site (reference
+        // src/simulator.cu:577, :603-606); cubble_rescale_box takes it to the
+        // target gas fraction later.  The volume multiplier does not depend on
+        // the box (phi V_box = the bubbles' volume), and the flow box is kept
+        // relative to the box.
+        const double d = (double)(float)(2.0 * cfg.r_mean);
+        for (int k = 0; k < D; ++k) {
+            c.tfr[k] = c.lbb[k] + d * bpd[k];
+            c.interval[k] = c.tfr[k] - c.lbb[k];
+            c.flow_lbb[k] = c.interval[k] * cfg.flow_lbb[k] + c.lbb[k];
+            c.flow_tfr[k] = c.interval[k] * cfg.flow_tfr[k] + c.lbb[k];
+        }
+    }
+    if (1 == cfg.placement || 2 == cfg.placement) {
         // the reference's lattice (assignDataToBubbles, src/kernels.cu:1034-1056):
         // site i at ((i mod bx) / bx, ...) of the box, plus an offset of length
         // R w along the normalised vector of D uniforms (so in the positive

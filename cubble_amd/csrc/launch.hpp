@@ -52,6 +52,10 @@ void launch_set_limits(const Grid &g, long long add);
 // become the old velocities.  Needs zero velocities (a new foam).
 void launch_bootstrap(const Grid &g, bool mixed);
 
+// Positions of one foam moved from its box (h.c) to the box of new_c, each
+// coordinate keeping its relative place; the constants are the caller's.
+void launch_rescale(const Foam &h, Foam *dev, const Consts &new_c, hipStream_t stream);
+
 // One foam's observables (h: a current host copy of its header, dev: its
 // device header): out[0..7] = total energy, total volume, then the sums of
 // r, path, dxdt, dydt, dzdt, drdt; energy[i] (if not null) = bubble i's

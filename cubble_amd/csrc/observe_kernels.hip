@@ -81,7 +81,24 @@ __global__ __launch_bounds__(kBlock) void k_observe_fin(Foam *F, int nb, double 
     }
 }
 
+// Box rescale (reference transformPositions, src/kernels.cu:970-990): each
+// coordinate to its relative position in the old box, then into the new one.
+__global__ __launch_bounds__(kBlock) void k_rescale(Foam *F, Consts old_c, Consts new_c) {
+    const Roles R = F->r;
+    const int n = F->d.count, D = old_c.dim;
+    for (int i = blockIdx.x * kBlock + (int)threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int a = 0; a < D; ++a) {
+            const double rel = (R.x[a][i] - old_c.lbb[a]) / old_c.interval[a];
+            R.x[a][i] = new_c.interval[a] * rel + new_c.lbb[a];
+        }
+}
+
 }  // namespace
+
+void launch_rescale(const Foam &h, Foam *dev, const Consts &new_c, hipStream_t stream) {
+    const int nb = std::max(1, std::min(4096, (h.d.count + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_rescale, dim3(nb), dim3(kBlock), 0, stream, dev, h.c, new_c);
+}
 
 void launch_observe(const Foam &h, Foam *dev, double *out, double *energy, hipStream_t stream) {
     const int nb = std::max(1, std::min(h.nblk, (h.d.count + kBlock * h.ipt - 1) / (kBlock * h.ipt)));

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -390,6 +391,46 @@ int cubble_reset_print_window(cubble_sim *s) {
     h.d.ts_max = kTsMaxEmpty;
     h.d.ts_sum = 0.0;
     return e.write_dyn(0, h.d);
+}
+
+int cubble_rescale_box(cubble_sim *s, double phi, const double rel_dim[3]) {
+    if (!s || !rel_dim || !(phi > 0.0)) return CUBBLE_E_INVALID;
+    Engine &e = s->e;
+    cubble_observables o;
+    int rc = cubble_observe(s, &o, nullptr);
+    if (rc) return rc;
+    Foam h;
+    if ((rc = e.read_header(0, h))) return rc;
+    const int D = h.c.dim;
+    // reference src/simulator.cu:800-826
+    double t = o.total_volume / (phi * rel_dim[0] * rel_dim[1]);
+    if (3 == D) {
+        t /= rel_dim[2];
+        t = std::cbrt(t);
+    } else {
+        t = std::sqrt(t);
+    }
+    Consts c = h.c;
+    for (int k = 0; k < 3; ++k) {
+        c.tfr[k] = k < D ? t * rel_dim[k] : 0.0;
+        c.interval[k] = c.tfr[k] - c.lbb[k];
+        // the flow box keeps its place relative to the box
+        if (k < D) {
+            c.flow_lbb[k] = c.interval[k] * ((h.c.flow_lbb[k] - h.c.lbb[k]) / h.c.interval[k]) + c.lbb[k];
+            c.flow_tfr[k] = c.interval[k] * ((h.c.flow_tfr[k] - h.c.lbb[k]) / h.c.interval[k]) + c.lbb[k];
+        }
+    }
+    double vbox = c.interval[0] * c.interval[1];
+    if (3 == D) vbox *= c.interval[2];
+    const double mult = phi * vbox / kPi;
+    c.vol_mul = 3 == D ? std::cbrt(0.75 * mult) : std::sqrt(mult);
+    launch_rescale(h, e.dfoams(), c, e.stream);
+    R_TRY(e, hipGetLastError());
+    R_TRY(e, hipStreamSynchronize(e.stream));
+    R_TRY(e, hipMemcpy(&e.dfoams()[0].c, &c, sizeof(Consts), hipMemcpyHostToDevice));
+    e.foams[0].h.c = c;
+    launch_set_flags(e.grid(), 1, -1, 1);
+    return e.search_pending();
 }
 
 int cubble_bootstrap_velocities(cubble_sim *s) {

@@ -107,9 +107,12 @@ typedef struct cubble_config {
                                  reference's lattice (spacing 2 R, a random
                                  offset of up to R per bubble, then scaled to
                                  the box; src/simulator.cu:576-606,
-                                 src/kernels.cu:1026-1064).  With 1, num_start
-                                 must be the lattice count (the JSON reader
-                                 rounds numStart up to it, as the reference)   */
+                                 src/kernels.cu:1026-1064); 2 = that lattice
+                                 in its own box (edge 2 R per site), for
+                                 cubble_rescale_box later.  With 1 or 2,
+                                 num_start must be the lattice count (the JSON
+                                 reader rounds numStart up to it, as the
+                                 reference)                                    */
 } cubble_config;
 
 /* State view: caller-owned host buffers, filled by cubble_get_state or read
@@ -265,6 +268,14 @@ int cubble_set_stop_rules(cubble_sim *sim, const cubble_stop_rules *rules);
  * :1127-1131): the per-bubble error array, and the window of time steps that
  * cubble_scalars.ts_min / ts_max / ts_sum report. */
 int cubble_reset_print_window(cubble_sim *sim);
+
+/* The box rescale of the reference's init (src/simulator.cu:793-846): the
+ * box becomes t * rel_dim (rel_dim[2] ignored in 2-D) with t chosen so the
+ * bubbles' volume fills the fraction phi; every coordinate keeps its relative
+ * place, the flow box likewise, the volume multiplier follows the new box,
+ * and the neighbour lists are searched again.  For foams created with
+ * placement 2 (the lattice in its own box). */
+int cubble_rescale_box(cubble_sim *sim, double phi, const double rel_dim[3]);
 
 /* The AB2 velocity bootstrap of the reference's init (src/simulator.cu:
  * 739-787): the pair velocities (no gas exchange, flow or walls) at the

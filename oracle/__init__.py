@@ -159,6 +159,7 @@ def lib():
         L.ora_init_synthetic.restype = vp
         L.ora_observe.argtypes = [vp, _dp, _dp]
         L.ora_bootstrap.argtypes = [vp]
+        L.ora_rescale.argtypes = [vp, ctypes.c_double, _dp, ctypes.c_double]
         L.ora_cbrt.argtypes = [ctypes.c_double]
         L.ora_cbrt.restype = ctypes.c_double
         u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -361,6 +362,10 @@ class Foam:
 
     def bootstrap(self):
         lib().ora_bootstrap(self.h)
+
+    def rescale(self, phi, rel_dim, volume):
+        rel = np.ascontiguousarray(rel_dim, dtype=np.float64)
+        lib().ora_rescale(self.h, float(phi), _dptr(rel), float(volume))
 
     def search(self):
         lib().ora_search(self.h)

@@ -948,6 +948,42 @@ void ora_observe(void *p, double out[8], double *energy) {
     }
 }
 
+/* The box rescale of init (reference src/simulator.cu:793-846, kernel
+ * transformPositions src/kernels.cu:970-990): box t * rel_dim with t from
+ * the bubbles' total volume `vol` and phi, coordinates and flow box kept
+ * relative, the volume multiplier from the new box, then a search. */
+void ora_rescale(void *p, double phi, const double rel[3], double vol) {
+    Foam &f = *static_cast<Foam *>(p);
+    ora_consts &c = f.c;
+    const int D = c.dim;
+    double t = vol / (phi * rel[0] * rel[1]);
+    if (3 == D) {
+        t /= rel[2];
+        t = std::cbrt(t);
+    } else {
+        t = std::sqrt(t);
+    }
+    const ora_consts old = c;
+    for (int k = 0; k < 3; ++k) {
+        c.tfr[k] = k < D ? t * rel[k] : 0.0;
+        c.interval[k] = c.tfr[k] - c.lbb[k];
+        if (k < D) {
+            c.flow_lbb[k] = c.interval[k] * ((old.flow_lbb[k] - old.lbb[k]) / old.interval[k]) + c.lbb[k];
+            c.flow_tfr[k] = c.interval[k] * ((old.flow_tfr[k] - old.lbb[k]) / old.interval[k]) + c.lbb[k];
+        }
+    }
+    for (int k = 0; k < D; ++k)
+        for (int32_t i = 0; i < f.n; ++i) {
+            const double r = (f.x[k][i] - old.lbb[k]) / old.interval[k];
+            f.x[k][i] = c.interval[k] * r + c.lbb[k];
+        }
+    double vbox = c.interval[0] * c.interval[1];
+    if (3 == D) vbox *= c.interval[2];
+    const double mult = phi * vbox / kPi;
+    c.vol_mul = 3 == D ? std::cbrt(0.75 * mult) : std::sqrt(mult);
+    search(f);
+}
+
 /* The AB2 velocity bootstrap of init (reference src/simulator.cu:739-787,
  * kernel euler src/kernels.cu:958-968): with the predicted state equal to
  * the current one, the pair velocities v0 (no gas exchange, no flow, no

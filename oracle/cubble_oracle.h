@@ -131,6 +131,10 @@ void ora_observe(void *f, double out[8], double *energy);
 /* The AB2 velocity bootstrap of init: old velocities = pair velocities
  * after one Euler step of the pair velocities (no gas, flow or walls). */
 void ora_bootstrap(void *f);
+
+/* The box rescale of init to gas fraction phi (box t * rel), given the
+ * bubbles' total volume; then a search. */
+void ora_rescale(void *f, double phi, const double rel[3], double vol);
 /* Deterministic cube root used by the volume fix. */
 double ora_cbrt(double a);
 /* Raw outputs of the project RNG, for known-answer tests: SplitMix64 from

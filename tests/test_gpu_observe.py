@@ -339,3 +339,30 @@ def test_cli_runs_the_imposed_flow(tmp_path, monkeypatch, capfd):
         means[impose] = float(rows[-1][6])   # mean dx/dt
     capfd.readouterr()
     assert means[1] > means[0] + 0.05
+
+
+def test_cli_lattice_start_scales_the_box(tmp_path, monkeypatch, capfd):
+    src = os.path.join(os.path.dirname(__file__), "golden", "input_small_2d.json")
+    with open(src) as fh:
+        j = json.load(fh)
+    j["init"]["placement"] = "grid"
+    j["stabilization"] = {"steps": 10, "maxDeltaEnergy": 1e9}
+    j["run"]["maxSteps"] = 100
+    path = str(tmp_path / "input.json")
+    with open(path, "w") as fh:
+        json.dump(j, fh)
+    monkeypatch.chdir(tmp_path)
+    capfd.readouterr()
+    assert ca.run_cli(["cubble-cli", path]) == 0
+    ctypes.CDLL(None).fflush(None)
+    out = capfd.readouterr().out.splitlines()
+    i = out.index("Scaling the simulation box")
+    assert out.index("Stabilizing a few rounds after creation") < i
+    first, second = out[i + 1], out[i + 2]
+    assert first.startswith("Current phi: ") and second.startswith("Current phi: ")
+    phi0 = float(first.split()[2].rstrip(","))
+    phi1 = float(second.split()[2].rstrip(","))
+    assert phi0 < 0.85 and phi1 == pytest.approx(0.9, abs=1e-9)
+    assert out[i + 3] == "Neighbor search after scaling"
+    assert out[i + 4] == "Stabilizing a few rounds after scaling"
+    assert out[-1] == "Done"

@@ -361,3 +361,33 @@ def test_velocity_bootstrap_matches_oracle(ora, dim, n, seed, precision):
         sim.bootstrap_velocities()
     assert e.value.code == ca.E_STATE
     sim.close()
+
+
+@pytest.mark.parametrize("dim,n,seed,rel", [(2, 2025, 81, (1.0, 1.0, 1.0)),
+                                            (3, 4096, 82, (1.0, 1.0, 1.0)),
+                                            (2, 2048, 83, (2.0, 1.0, 1.0))])
+def test_rescale_box_matches_oracle(ora, dim, n, seed, rel):
+    """A lattice in its own box (placement 2), scaled to phi: positions, box,
+    flow box and volume multiplier equal the oracle's given the same total
+    volume; the search that follows equals it bit for bit."""
+    print(f"rescale dim={dim} n={n} seed={seed} rel={rel}", flush=True)
+    cfg = config(dim, n, seed=seed, placement=2, rel_dim=rel, phi=0.85)
+    sim = ca.Simulator(cfg)
+    before = sim.scalars()
+    f = oracle_from_sim(ora, sim)
+    vol = sim.observe()["total_volume"]
+    assert vol / np.prod(before["interval"][:dim]) < 0.8   # the lattice box is loose
+    sim.rescale_box(0.85, rel)
+    f.rescale(0.85, rel, vol)
+    gst, gsc = sim.get_state()
+    ost, _ = f.get_state()
+    oc = f.consts()
+    assert np.array_equal(gst["x"], ost["x"])
+    for k in ("interval", "tfr", "flow_lbb", "flow_tfr"):
+        assert list(gsc[k][:dim]) == list(oc[k][:dim]), k
+    assert gsc["vol_mul"] == oc["vol_mul"]
+    assert_same_structure(gst, ost)
+    assert vol / np.prod(gsc["interval"][:dim]) == pytest.approx(0.85, rel=1e-12)
+    ratio = np.array(gsc["interval"][:dim]) / np.array(rel[:dim])
+    assert np.allclose(ratio, ratio[0], rtol=1e-14)
+    sim.close()
