@@ -399,6 +399,7 @@ int cubble_rescale_box(cubble_sim *s, double phi, const double rel_dim[3]) {
     cubble_observables o;
     int rc = cubble_observe(s, &o, nullptr);
     if (rc) return rc;
+    if (!(o.total_volume > 0.0)) return e.fail(CUBBLE_E_STATE, "rescale_box: no bubble volume");
     Foam h;
     if ((rc = e.read_header(0, h))) return rc;
     const int D = h.c.dim;

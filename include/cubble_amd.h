@@ -274,7 +274,8 @@ int cubble_reset_print_window(cubble_sim *sim);
  * bubbles' volume fills the fraction phi; every coordinate keeps its relative
  * place, the flow box likewise, the volume multiplier follows the new box,
  * and the neighbour lists are searched again.  For foams created with
- * placement 2 (the lattice in its own box). */
+ * placement 2 (the lattice in its own box).  CUBBLE_E_INVALID for phi <= 0,
+ * CUBBLE_E_STATE for a foam without bubble volume. */
 int cubble_rescale_box(cubble_sim *sim, double phi, const double rel_dim[3]);
 
 /* The AB2 velocity bootstrap of the reference's init (src/simulator.cu:
