@@ -1,6 +1,7 @@
 """CPU-only checks of the product library: it loads, exports every symbol the
 C header declares, its host-side initial foam equals the oracle's bit for bit,
 and it reads the reference's input_parameters.json schema.  No GPU compute."""
+import ctypes
 import os
 import re
 
@@ -144,3 +145,17 @@ def test_grid_placement_positions(dim, bpd):
     assert vol.sum() == pytest.approx(0.9 * np.prod(L), rel=1e-12)
     # not all offsets vanish
     assert np.std(d) > 0.1 * scale.min()
+
+
+def test_new_entry_points_reject_a_null_handle():
+    """Argument checks come before any device work, so they run without a GPU."""
+    L = ca.lib()
+    obs = ca._Observables()
+    rules = ca._StopRules()
+    rel = np.ones(3)
+    assert L.cubble_observe(None, ctypes.byref(obs), None) == ca.E_INVALID
+    assert L.cubble_set_stop_rules(None, ctypes.byref(rules)) == ca.E_INVALID
+    assert L.cubble_reset_print_window(None) == ca.E_INVALID
+    assert L.cubble_reset_clock(None, 1) == ca.E_INVALID
+    assert L.cubble_bootstrap_velocities(None) == ca.E_INVALID
+    assert L.cubble_rescale_box(None, 0.9, rel.ctypes.data_as(ca._dp)) == ca.E_INVALID
