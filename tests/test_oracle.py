@@ -612,3 +612,34 @@ def test_bootstrap_two_bubble_closed_form(ora):
     assert got["vo"][0, 2] == 0.0 and not np.any(got["vo"][1])
     assert not np.any(got["v"])
     assert np.array_equal(got["x"], st["x"])
+
+
+# --------------------------------------------------------------------------
+# the box rescale of init
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,rel,phi", [(2, (2.0, 1.0, 1.0), 0.7), (3, (1.0, 1.5, 1.0), 0.8)])
+def test_rescale_keeps_relative_positions_and_fills_phi(ora, dim, rel, phi):
+    f = ora.Foam.synthetic(dim=dim, count=900, phi=0.5, seed=31, radius_dist=dim - 2, r_std=0.15)
+    st0, _ = f.get_state()
+    c0 = f.consts()
+    vol = f.observe()[0][1]
+    f.rescale(phi, rel, vol)
+    st1, _ = f.get_state()
+    c1 = f.consts()
+    L0 = np.array(c0["interval"][:dim])
+    L1 = np.array(c1["interval"][:dim])
+    # the box holds phi, in the proportions rel
+    assert vol / np.prod(L1) == pytest.approx(phi, rel=1e-13)
+    assert np.allclose(L1 / np.array(rel[:dim]), L1[0] / rel[0], rtol=1e-14)
+    # every bubble keeps its relative place (matched by original index)
+    o0 = np.argsort(st0["idx"])
+    o1 = np.argsort(st1["idx"])
+    rel0 = (st0["x"][:dim, o0] - np.array(c0["lbb"][:dim])[:, None]) / L0[:, None]
+    rel1 = (st1["x"][:dim, o1] - np.array(c1["lbb"][:dim])[:, None]) / L1[:, None]
+    assert np.allclose(rel0, rel1, rtol=0, atol=1e-14)
+    # volume multiplier: M = sqrt(phi V_box / pi) or cbrt(3/4 phi V_box / pi)
+    m = phi * np.prod(L1) / PI
+    want = np.sqrt(m) if dim == 2 else np.cbrt(0.75 * m)
+    assert c1["vol_mul"] == pytest.approx(want, rel=1e-14)
+    # the lists of the new box are the brute-force pairs
+    assert np.array_equal(f.pairs(), f.brute_pairs())
