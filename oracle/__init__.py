@@ -160,6 +160,7 @@ def lib():
         L.ora_observe.argtypes = [vp, _dp, _dp]
         L.ora_bootstrap.argtypes = [vp]
         L.ora_rescale.argtypes = [vp, ctypes.c_double, _dp, ctypes.c_double]
+        L.ora_wall_velocity.argtypes = [ctypes.POINTER(Consts), _dp, ctypes.c_double, _dp]
         L.ora_cbrt.argtypes = [ctypes.c_double]
         L.ora_cbrt.restype = ctypes.c_double
         u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -401,6 +402,16 @@ def half_pairs(off, nbr):
 
 def cbrt(a):
     return lib().ora_cbrt(float(a))
+
+
+def wall_velocity(consts, pos, rad, v):
+    """The oracle's wall term alone: velocity v (3) of a bubble at pos (3)
+    with radius rad, after the walls of `consts` (a dict)."""
+    c = consts_from_dict(consts)
+    p = np.array(list(pos) + [0.0] * (3 - len(pos)), float)
+    out = np.array(list(v) + [0.0] * (3 - len(v)), float)
+    lib().ora_wall_velocity(ctypes.byref(c), _dptr(p), float(rad), _dptr(out))
+    return out
 
 
 def splitmix64(seed, n):

@@ -1006,6 +1006,10 @@ void ora_bootstrap(void *p) {
 
 double ora_cbrt(double a) { return det_cbrt(a); }
 
+void ora_wall_velocity(const ora_consts *c, const double pos[3], double rad, double v[3]) {
+    add_wall_velocity(*c, pos[0], pos[1], pos[2], rad, v[0], v[1], v[2]);
+}
+
 void ora_splitmix64(uint64_t seed, int32_t n, uint64_t *out) {
     uint64_t x = seed;
     for (int32_t k = 0; k < n; ++k) out[k] = Rng::splitmix(x);

@@ -135,6 +135,9 @@ void ora_bootstrap(void *f);
 /* The box rescale of init to gas fraction phi (box t * rel), given the
  * bubbles' total volume; then a search. */
 void ora_rescale(void *f, double phi, const double rel[3], double vol);
+/* The wall term of the corrector (src/kernels.cu:546-593) on its own:
+ * v (x, y, z) is updated in place for a bubble at pos with radius rad. */
+void ora_wall_velocity(const ora_consts *c, const double pos[3], double rad, double v[3]);
 /* Deterministic cube root used by the volume fix. */
 double ora_cbrt(double a);
 /* Raw outputs of the project RNG, for known-answer tests: SplitMix64 from

@@ -643,3 +643,183 @@ def test_rescale_keeps_relative_positions_and_fills_phi(ora, dim, rel, phi):
     assert c1["vol_mul"] == pytest.approx(want, rel=1e-14)
     # the lists of the new box are the brute-force pairs
     assert np.array_equal(f.pairs(), f.brute_pairs())
+
+
+# --------------------------------------------------------------------------
+# walls (reference addWallVelocity, src/kernels.cu:546-593)
+#
+# Durian's wall term is the pair term of the bubble's mirror image: a bubble
+# at distance d < r from a wall overlaps its reflection (same radius, at 2d)
+# and f0 (1/(2d) - 1/(2r)) 2d = f0 (1 - d/r), directed away from the wall.
+# A touched wall drags the velocity components along it by (1 - drag), so a
+# component k ends as (v_k + push_k) * (1 - drag)^(walls touched other than
+# k).  "Touching" includes d == r exactly (push 0, drag applied).
+# --------------------------------------------------------------------------
+WALL_F0, WALL_DRAG = 1.7, 0.3
+
+
+def _wall_consts(dim, wall, L=10.0):
+    return consts(dim, L=L, wall=wall, f0=WALL_F0, drag=WALL_DRAG)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("side", ["low", "high"])
+def test_wall_single_wall_closed_form(ora, dim, side):
+    L, r, d = 10.0, 0.8, 0.35
+    v0 = np.array([0.11, -0.07, 0.05])[:dim]
+    keep = 1.0 - WALL_DRAG
+    for a in range(dim):
+        wall = [0, 0, 0]
+        wall[a] = 1
+        pos = np.full(dim, 4.0)
+        pos[a] = d if side == "low" else L - d
+        got = ora.wall_velocity(_wall_consts(dim, wall, L), pos, r, v0)[:dim]
+        push = WALL_F0 * (1.0 - d / r) * (1.0 if side == "low" else -1.0)
+        want = v0 * keep
+        want[a] = v0[a] + push
+        assert np.allclose(got, want, rtol=1e-14, atol=0), (a, side, got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("r,d", [(1.0, 0.3), (0.7, 0.65), (1.3, 0.05)])
+def test_wall_push_equals_mirror_image_pair_velocity(ora, dim, r, d):
+    # wall box: one bubble at distance d from the x = 0 wall
+    L = 10.0
+    got = ora.wall_velocity(_wall_consts(dim, [1, 0, 0], L), [d, 4.0, 4.0][:dim], r,
+                            np.zeros(dim))[0]
+    # periodic box: the same bubble and its mirror image at distance 2 d,
+    # through the oracle's pair step (pinned by the two-bubble closed form)
+    pos = [[5.0 + d, 5.0, 5.0], [5.0 - d, 5.0, 5.0]]
+    f = foam(ora, pos, [r, r], dim, L=20.0, f0=WALL_F0, dt=1e-9, tol=1e9)
+    f.attempt(gas=False, path=False)
+    st, _ = f.get_state()
+    k = int(np.where(st["idx"] == 0)[0][0])
+    mirror = f.attempt_out()["vp"][0, k]
+    assert got == pytest.approx(mirror, rel=1e-14)
+    assert got == pytest.approx(WALL_F0 * (1.0 - d / r), rel=1e-14)
+
+
+@pytest.mark.parametrize("dim,walls", [(2, (1, 1, 0)), (3, (1, 1, 0)), (3, (1, 0, 1)),
+                                       (3, (0, 1, 1)), (3, (1, 1, 1))])
+def test_wall_corner_drag_law(ora, dim, walls):
+    L, r = 10.0, 0.9
+    dist = np.array([0.2, 0.5, 0.7])  # distance to the touched wall per axis
+    pos = np.full(3, 5.0)
+    low = np.array([True, False, True])
+    for a in range(dim):
+        if walls[a]:
+            pos[a] = dist[a] if low[a] else L - dist[a]
+    v0 = np.array([0.21, -0.13, 0.08])
+    got = ora.wall_velocity(_wall_consts(dim, list(walls), L), pos, r, v0)
+    keep = 1.0 - WALL_DRAG
+    touched = [a for a in range(dim) if walls[a]]
+    for k in range(dim):
+        push = WALL_F0 * (1.0 - dist[k] / r) * (1.0 if low[k] else -1.0) if walls[k] else 0.0
+        others = sum(1 for a in touched if a != k)
+        want = (v0[k] + push) * keep ** others
+        assert got[k] == pytest.approx(want, rel=1e-14), (k, got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wall_not_touching_and_touching_at_contact(ora, dim):
+    L, r = 10.0, 0.5
+    v0 = np.array([0.3, -0.2, 0.1])[:dim]
+    c = _wall_consts(dim, [1, 1, 1], L)
+    # farther than r from every wall: unchanged, bit for bit
+    got = ora.wall_velocity(c, np.full(dim, 5.0), r, v0)[:dim]
+    assert np.array_equal(got, v0)
+    far = ora.wall_velocity(c, [r + 1e-9] + [5.0] * (dim - 1), r, v0)[:dim]
+    assert np.array_equal(far, v0)
+    # exactly r from the x = 0 wall (1.5 - 1.0 is exact): no push, drag applies
+    at = ora.wall_velocity(_wall_consts(dim, [1, 1, 1], L), [0.5] + [5.0] * (dim - 1), r, v0)[:dim]
+    want = v0 * (1.0 - WALL_DRAG)
+    want[0] = v0[0]
+    assert np.allclose(at, want, rtol=1e-15, atol=0)
+
+
+def test_wall_velocity_through_an_attempt(ora):
+    # one bubble near the lower x wall and the upper y wall, zero history:
+    # the predicted velocity is the wall term alone, and the corrector moves
+    # the bubble by dt/2 times it (trapezoid with zero current velocity)
+    L, r, dt = 10.0, 0.8, 1e-3
+    pos = [0.5, L - 0.6, 5.0]
+    f = foam(ora, [pos], [r], 3, L=L, wall=(1, 1, 0), f0=WALL_F0, drag=WALL_DRAG, dt=dt, tol=1e9)
+    f.attempt(gas=False, path=False)
+    o = f.attempt_out()
+    keep = 1.0 - WALL_DRAG
+    ux = WALL_F0 * (1.0 - 0.5 / r)
+    uy = -WALL_F0 * (1.0 - 0.6 / r)
+    want = np.array([ux * keep, uy * keep, 0.0])
+    assert np.allclose(o["vp"][:3, 0], want, rtol=1e-14, atol=0)
+    assert np.allclose(o["xc"][:3, 0], np.array(pos) + 0.5 * dt * want, rtol=0, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# expansion since the last search and the search trigger
+# (reference src/kernels.cu:369-387, :412, :487-495; src/simulator.cu:323)
+#
+# A free bubble (no neighbours, no walls) has zero predicted velocity, so the
+# trapezoid gives x_c = x + dt/2 v, r_c = r + dt/2 (dr/dt + gas term), and its
+# expansion is |x_c - x_saved| + (r_c - r_saved).  Equal radii make the
+# liquid-mediated gas term zero (1/rho = 1/r).  A search follows an accepted
+# step whose largest expansion is at least half the skin.
+# --------------------------------------------------------------------------
+def _free_foam(ora, pos, r, v, saved, dt, skin=0.3, gas_drdt=0.0):
+    n = len(r)
+    vv = np.zeros((4, n))
+    vv[:3] = np.asarray(v, float).T
+    vv[3] = gas_drdt
+    f = foam(ora, pos, r, 3, v=vv, dt=dt, tol=1e9, skin=skin)
+    st, sc = f.get_state()
+    order = st["idx"]
+    st["saved"] = np.asarray(saved, float).T[:, order].copy()
+    f.set_state(st, sc)
+    return f, order
+
+
+def test_expansion_free_bubbles_closed_form(ora):
+    dt, drdt = 0.2, 0.05
+    pos = np.array([[3.0, 3.0, 3.0], [10.0, 10.0, 10.0], [16.0, 4.0, 12.0]])
+    r = np.array([1.0, 1.0, 1.0])
+    v = np.array([[0.3, -0.4, 0.1], [0.05, 0.02, -0.9], [-0.2, 0.2, 0.2]])
+    saved = np.column_stack([pos - np.array([[0.01, 0.02, -0.03], [0.0, 0.0, 0.0],
+                                             [0.04, -0.01, 0.02]]),
+                             r - np.array([0.002, 0.03, -0.01])])
+    f, order = _free_foam(ora, pos, r, v, saved, dt, gas_drdt=drdt)
+    f.attempt(gas=True, path=False)
+    xc = pos + 0.5 * dt * v
+    rc = r + 0.5 * dt * drdt
+    exp = np.linalg.norm(xc - saved[:, :3], axis=1) + (rc - saved[:, 3])
+    assert f.scalars()["max_exp"] == pytest.approx(exp.max(), rel=1e-12)
+    # the bubble with the largest expansion decides (bubble 1: it moved most)
+    assert int(np.argmax(exp)) == 1
+    # without gas exchange the radius term drops out
+    g, _ = _free_foam(ora, pos, r, v, saved, dt, gas_drdt=drdt)
+    g.attempt(gas=False, path=False)
+    assert g.scalars()["max_exp"] == pytest.approx(
+        np.linalg.norm(xc - saved[:, :3], axis=1).max(), rel=1e-12)
+
+
+@pytest.mark.parametrize("saved_x,want", [(0.25, 1), (0.25 + 2.0 ** -54, 0)])
+def test_search_trigger_at_half_skin(ora, saved_x, want):
+    # skin 0.5: a bubble at rest at x = 0.5, its saved position 0.25 (exactly
+    # half the skin away) or one ulp above it (0.25 - 2^-54 away: both values
+    # and their difference are exact doubles); no gas exchange, so no radius
+    # term, and sqrt(t * t) == t
+    skin = 0.5
+    pos = np.array([[0.5, 5.0, 5.0]])
+    saved = np.array([[saved_x, 5.0, 5.0, 1.0]])
+    f, _ = _free_foam(ora, pos, [1.0], np.zeros((1, 3)), saved, dt=1e-3, skin=skin)
+    f.attempt(gas=False, path=False)
+    sc = f.scalars()
+    assert sc["max_exp"] == 0.5 - saved_x
+    assert (sc["max_exp"] >= 0.25) == bool(want)
+    assert sc["need_search"] == want
+    g, _ = _free_foam(ora, pos, [1.0], np.zeros((1, 3)), saved, dt=1e-3, skin=skin)
+    before = g.scalars()["searches"]
+    g.integrate(gas=False, path=False)
+    assert g.scalars()["searches"] - before == want
+    # a search resets the saved positions to the current ones
+    if want:
+        st, _ = g.get_state()
+        assert np.array_equal(st["saved"][:3, 0], st["x"][:3, 0])
