@@ -301,7 +301,7 @@ class Config:
     dim: int = 2
     num_start: int = 100000
     num_end: int = 1000
-    radius_dist: int = 0          # 0 clipped normal, 1 log-normal
+    radius_dist: int = 0          # 0 clipped normal, 1 log-normal, 2 |normal| (+minRad)
     r_mean: float = 1.0
     r_std: float = 0.21
     r_lo: float = 0.5

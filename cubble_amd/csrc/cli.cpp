@@ -4,9 +4,13 @@
 // phase of init() (src/simulator.cu:739-791, :848-955) and simulate()
 // (src/simulator.cu:958-1157): argument check, device check, stabilisation
 // with its table, the integration loop and its stop rule, the printed table,
-// results.dat and the snapshot CSVs.  The foam is the synthetic one drawn at
-// the target gas fraction; the reference's grid placement and box rescale
-// are not repeated.
+// results.dat and the snapshot CSVs.  A lattice start (the default of a file
+// without an "init" object, as the reference's own input) follows init: the
+// lattice in its own box, the velocity bootstrap, five stabilisation rounds,
+// the box rescale to the target gas fraction with a new search, five more
+// rounds (src/simulator.cu:576-606, :739-846).  A uniform start ("init":
+// {"placement": "uniform"}) is drawn at the target gas fraction directly and
+// skips the rescale.
 //
 // The reference's host loop checks after every accepted step whether to
 // print, snapshot or stop.  Here the steps run on the device

@@ -329,6 +329,10 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
         err = "placement must be 0 (uniform), 1 (grid) or 2 (grid in the lattice box)";
         return CUBBLE_E_INVALID;
     }
+    if (cfg.radius_dist < 0 || cfg.radius_dist > 2) {
+        err = "radius_dist must be 0 (clipped normal), 1 (log-normal) or 2 (the reference's |normal|)";
+        return CUBBLE_E_INVALID;
+    }
     if (!(cfg.phi > 0.0 && cfg.phi < 1.0) || !(cfg.r_mean > 0.0) || !(cfg.mu > 0.0)) {
         err = "phi must be in (0, 1) and radius.mean, mu positive";
         return CUBBLE_E_INVALID;
@@ -341,6 +345,12 @@ int generate_foam(const cubble_config &cfg, std::vector<double> &x4, InitOut &ou
         double v;
         if (1 == cfg.radius_dist) {
             v = cfg.r_mean * std::exp(cfg.r_std * z);
+        } else if (2 == cfg.radius_dist) {
+            // the reference's rule (assignDataToBubbles, src/kernels.cu:1051-1053):
+            // |N(mean, std)|, plus minRad (0.1 mean) when at or below minRad
+            const double min_rad = 0.1 * cfg.r_mean;
+            v = std::fabs(cfg.r_mean + cfg.r_std * z);
+            v = v > min_rad ? v : v + min_rad;
         } else {
             v = cfg.r_mean + cfg.r_std * z;
             v = v < cfg.r_lo ? cfg.r_lo : (v > cfg.r_hi ? cfg.r_hi : v);
@@ -491,13 +501,23 @@ int parse_config_json(const char *path, cubble_config &cfg, std::string &err) {
     }
     cfg.r_lo = 0.5 * cfg.r_mean;
     cfg.r_hi = 1.5 * cfg.r_mean;
-    // keys this build adds (ignored by the reference, README.md:66-69)
+    // keys this build adds (ignored by the reference, README.md:66-69).  A
+    // file without an "init" object is the reference's own input: its init
+    // places the bubbles on the lattice and takes the radii as |N(mean, std)|
+    // raised by minRad at or below minRad (src/simulator.cu:576-606,
+    // src/kernels.cu:1034-1053).  With an "init" object the defaults are
+    // uniform placement and the normal clipped to radiusClip.
+    if (!root.get("init")) {
+        cfg.placement = 1;
+        cfg.radius_dist = 2;
+    }
     std::string s;
     if (jstr(root, {"init", "radiusDistribution"}, s)) {
         if (s == "lognormal") cfg.radius_dist = 1;
         else if (s == "normal") cfg.radius_dist = 0;
+        else if (s == "reference") cfg.radius_dist = 2;
         else {
-            err = "init.radiusDistribution must be \"normal\" or \"lognormal\"";
+            err = "init.radiusDistribution must be \"normal\", \"lognormal\" or \"reference\"";
             return CUBBLE_E_INVALID;
         }
     }

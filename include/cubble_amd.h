@@ -84,7 +84,12 @@ typedef struct cubble_config {
     int32_t num_start;        /* bubbles.numStart                            */
     int32_t num_end;          /* bubbles.numEnd (CLI stop rule)              */
     int32_t radius_dist;      /* 0 = normal clipped to [r_lo, r_hi],
-                                 1 = log-normal (median r_mean, sigma r_std) */
+                                 1 = log-normal (median r_mean, sigma r_std),
+                                 2 = the reference's rule: |N(r_mean, r_std)|,
+                                 plus minRad (0.1 r_mean) when at or below
+                                 minRad (src/kernels.cu:1051-1053); the JSON
+                                 reader's default when the file has no
+                                 "init" object                              */
     double r_mean;            /* bubbles.radius.mean                         */
     double r_std;             /* bubbles.radius.std                          */
     double r_lo, r_hi;        /* clip bounds for the normal distribution     */

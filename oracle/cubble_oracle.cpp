@@ -849,6 +849,11 @@ void *ora_init_synthetic(const ora_init_params *p) {
         double v;
         if (1 == p->radius_dist) {
             v = p->r_mean * std::exp(p->r_std * z);
+        } else if (2 == p->radius_dist) {
+            /* src/kernels.cu:1051-1053: |r|, plus minRad when at or below it */
+            const double min_rad = 0.1 * p->r_mean;
+            v = std::fabs(p->r_mean + p->r_std * z);
+            v = v > min_rad ? v : v + min_rad;
         } else {
             v = p->r_mean + p->r_std * z;
             v = v < p->r_lo ? p->r_lo : (v > p->r_hi ? p->r_hi : v);

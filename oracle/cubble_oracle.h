@@ -89,7 +89,7 @@ typedef struct ora_attempt_out {
 typedef struct ora_init_params {
     int32_t dim;
     int32_t count;
-    int32_t radius_dist;    /* 0 = clipped normal, 1 = log-normal */
+    int32_t radius_dist;    /* 0 = clipped normal, 1 = log-normal, 2 = |normal| (+ minRad at or below minRad) */
     double r_mean;          /* normal mean, or log-normal median */
     double r_std;           /* normal sd, or log-normal sigma */
     double r_lo, r_hi;      /* clip bounds (normal only) */

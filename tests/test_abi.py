@@ -159,3 +159,61 @@ def test_new_entry_points_reject_a_null_handle():
     assert L.cubble_reset_clock(None, 1) == ca.E_INVALID
     assert L.cubble_bootstrap_velocities(None) == ca.E_INVALID
     assert L.cubble_rescale_box(None, 0.9, rel.ctypes.data_as(ca._dp)) == ca.E_INVALID
+
+
+# --------------------------------------------------------------------------
+# the reference's own input: no "init" object
+# --------------------------------------------------------------------------
+def test_reference_input_defaults_to_lattice_and_reference_radii(tmp_path):
+    # input_parameters.json has no "init" object: the reference's init places
+    # numStart (rounded up to the lattice) on the lattice and draws
+    # |N(mean, std)| radii (src/simulator.cu:576-606, src/kernels.cu:1034-1053)
+    j = {"bubbles": {"numStart": 1e5, "numEnd": 1000, "radius": {"mean": 1.0, "std": 0.21}},
+         "box": {"dimensionality": 2, "relativeDimensions": {"x": 1, "y": 1, "z": 1}},
+         "constants": {"phi": {"value": 0.9}}}
+    p = tmp_path / "input_parameters.json"
+    p.write_text(__import__("json").dumps(j))
+    c = ca.Config.from_json(str(p))
+    assert c.placement == 1 and c.radius_dist == 2
+    assert c.num_start == 317 * 317  # ceil(sqrt(1e5)) per axis
+    # an "init" object, even an empty one, keeps uniform placement and the
+    # clipped normal unless its keys say otherwise
+    j["init"] = {}
+    p.write_text(__import__("json").dumps(j))
+    c = ca.Config.from_json(str(p))
+    assert c.placement == 0 and c.radius_dist == 0 and c.num_start == 100000
+    j["init"] = {"radiusDistribution": "reference", "placement": "grid"}
+    p.write_text(__import__("json").dumps(j))
+    c = ca.Config.from_json(str(p))
+    assert c.placement == 1 and c.radius_dist == 2
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reference_radius_rule_on_drawn_values(dim):
+    """radius_dist 2 applies |v| and +minRad (v at or below minRad) to the
+    very normals the unclipped normal draws from the same seed."""
+    n, mean, std = 20000, 1.0, 0.6
+    base = dict(dim=dim, num_start=n, r_mean=mean, r_std=std, phi=0.5, seed=77)
+    raw, _ = ca.generate_foam(ca.Config(radius_dist=0, r_lo=-1e300, r_hi=1e300, **base))
+    got, sc = ca.generate_foam(ca.Config(radius_dist=2, **base))
+    v = raw[3]
+    min_rad = 0.1 * mean
+    assert sc["min_rad"] == min_rad
+    a = np.abs(v)
+    want = np.where(a > min_rad, a, a + min_rad)
+    assert np.array_equal(got[3], want)
+    # every branch was drawn: negative values, and |v| at or below minRad
+    assert np.count_nonzero(v < 0) > 100 and np.count_nonzero(a <= min_rad) > 50
+    assert np.all(got[3] > min_rad)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reference_radius_rule_matches_oracle_init(ora, dim):
+    n = 3000
+    cfg = ca.Config(dim=dim, num_start=n, radius_dist=2, r_std=0.6, phi=0.8, seed=5)
+    x4, sc = ca.generate_foam(cfg)
+    f = ora.Foam.synthetic(dim=dim, count=n, phi=0.8, seed=5, radius_dist=2, r_std=0.6)
+    st, _ = f.get_state()
+    order = np.argsort(st["idx"])
+    assert np.array_equal(st["x"][3][order], x4[3])
+    assert sc["avg_area_in"] == f.consts()["avg_area_in"]
