@@ -81,30 +81,66 @@ def step_bytes(dim, p, mixed=False):
     return 8 * 2 * c + vb * 4 * c + 8 * 2 + vb * 2 + 8 * p
 
 
-KERNEL_NAMES = {"predict": "k_predict", "pair": "k_pair", "correct": "k_correct"}
+KERNEL_NAMES = {"predict": "k_predict", "pair": "k_pair", "correct": "k_correct",
+                "search": "k_s_search"}
 
 
 def kernel_name(part, mixed):
     return "k_pair_mixed" if part == "pair" and mixed else KERNEL_NAMES[part]
 
 
-def roofline(kt, per_foam, mixed=False):
-    """Roofline of the dominant kernel of an attempt.  kt: kernel times (ms) of
-    one launch over all foams; per_foam: [(dim, count, pairs)]."""
-    times = {"predict": kt["predict_ms"], "pair": kt["pair_ms"], "correct": kt["correct_ms"]}
-    dom = max(times, key=times.get)
+def search_bytes(dim, p):
+    """Compulsory HBM bytes per bubble of the list kernel k_s_search: read
+    the bubble's {x, y, z, r} record once (the candidates' records are
+    neighbours' own records, L2 hits in cell order), write its upper list (p
+    entries on average) and one lower-count increment per entry."""
+    return 8 * (dim + 1) + 4 * p + 4 * p
+
+
+def roofline(kt, st, per_foam, steps, attempts, searches, groups, mixed=False):
+    """Roofline of the kernel with the largest share of a step.  kt: attempt
+    kernel times (ms) of one launch over all foams; st: search timings of one
+    search of every foam (cubble_time_search); per_foam: [(dim, count,
+    pairs)]; steps, attempts, searches: counts over the timed run (searches
+    summed over foams), groups: attempt groups replayed.  Shares: the attempt
+    kernels run once per attempt, the list kernel once per search (its time
+    for all foams scaled by the searched bubbles' share), plus the idle
+    launches of the phase kernels once per group."""
+    bubbles = sum(n for _, n, _ in per_foam)
+    nf = max(len(per_foam), 1)
+    times = {"predict": kt["predict_ms"], "pair": kt["pair_ms"], "correct": kt["correct_ms"],
+             "search": st["list_kernel_ms"]}
+    share = {k: times[k] * attempts / nf / max(steps, 1) for k in ("predict", "pair", "correct")}
+    share["search"] = st["list_kernel_ms"] * searches / nf / max(steps, 1)
+    dom = max(share, key=share.get)
     nbytes = 0.0
-    bubbles = 0
     for dim, n, pairs in per_foam:
-        nbytes += kernel_bytes(dim, pairs / max(n, 1), mixed)[dom] * n
-        bubbles += n
+        p = pairs / max(n, 1)
+        nbytes += (search_bytes(dim, p) if dom == "search" else kernel_bytes(dim, p, mixed)[dom]) * n
     achieved = nbytes / (times[dom] * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": kernel_name(dom, mixed), "bytes_per_launch": round(nbytes),
             "bubbles_per_launch": bubbles,
             "launch_ms": round(times[dom], 4),
-            "kernels_ms": {k: round(v, 4) for k, v in times.items()}}
+            "kernels_ms": {k: round(v, 4) for k, v in times.items()},
+            "ms_per_step_share": {k: round(v, 4) for k, v in share.items()},
+            "search": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+            "idle_phase_ms_per_group": round(st["idle_ms"], 4),
+            "idle_phase_ms_per_step": round(st["idle_ms"] * groups / max(steps, 1), 4)}
+
+
+def attempt_rooflines(kt, per_foam, mixed=False):
+    """HBM fraction of each attempt kernel (algorithmic bytes / launch time)."""
+    out = {}
+    for part in ("predict", "pair", "correct"):
+        nbytes = sum(kernel_bytes(d, pr / max(n, 1), mixed)[part] * n for d, n, pr in per_foam)
+        gbs = nbytes / (kt[part + "_ms"] * 1e-3) / 1e9
+        out[kernel_name(part, mixed)] = {"achieved": round(gbs, 1),
+                                         "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                         "bytes_per_launch": round(nbytes),
+                                         "launch_ms": round(kt[part + "_ms"], 4)}
+    return out
 
 
 def counters(before, after):
@@ -140,7 +176,12 @@ def leg_batch(ca, args, ip):
     work = sum(x["bubble_steps"] for x in s1) - sum(x["bubble_steps"] for x in s0)
     assert all(r["steps"] >= args.steps for r in res)
     kt = b.time_kernels(args.kernel_reps, ip)
-    rf = roofline(kt, [(x["dim"], x["count"], x["num_pairs"]) for x in s1], mixed=True)
+    st = b.time_search(args.kernel_reps)
+    per_foam = [(x["dim"], x["count"], x["num_pairs"]) for x in s1]
+    cn = counters(s0, s1)
+    groups = max(x["attempts"] for x in s1) - max(x["attempts"] for x in s0)
+    rf = roofline(kt, st, per_foam, args.steps, cn["attempts"], cn["searches"], groups, mixed=True)
+    rf["attempt_kernels"] = attempt_rooflines(kt, per_foam, mixed=True)
     live = sum(x["count"] for x in s1)
     pairs = sum(x["num_pairs"] for x in s1)
     rf["step_bytes_per_bubble"] = round(
@@ -189,7 +230,12 @@ def leg_foam(ca, args, ip):
     r1 = sim.scalars()
     work = r1["bubble_steps"] - r0["bubble_steps"]
     kt = sim.time_kernels(args.kernel_reps, ip)
-    rf = roofline(kt, [(3, r1["count"], r1["num_pairs"])])
+    st = sim.time_search(args.kernel_reps)
+    cn = counters([r0], [r1])
+    per_foam = [(3, r1["count"], r1["num_pairs"])]
+    rf = roofline(kt, st, per_foam, args.foam_steps, cn["attempts"], cn["searches"],
+                  cn["attempts"])
+    rf["attempt_kernels"] = attempt_rooflines(kt, per_foam)
     p = r1["num_pairs"] / max(r1["count"], 1)
     rf["step_bytes_per_bubble"] = round(step_bytes(3, p), 2)
     rf["step_gbs"] = round(step_bytes(3, p) * work / elapsed / 1e9, 1)
@@ -200,6 +246,90 @@ def leg_foam(ca, args, ip):
             "ms_per_step": elapsed / args.foam_steps * 1e3, "precision": "fp64",
             "pairs_per_bubble": round(p, 3), "final_count": r1["count"],
             "counters": counters([r0], [r1]), "roofline": rf}
+
+
+def cpu_info():
+    """The host's CPU model, logical CPUs, CPUs this process may run on, and
+    the OpenMP thread count in use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "available": avail,
+            "omp_threads": int(os.environ.get("OMP_NUM_THREADS") or avail or 1)}
+
+
+def median(v):
+    v = sorted(v)
+    return v[len(v) // 2]
+
+
+def leg_foam2d(ca, args, ip):
+    """The 2D foam of 1e4 bubbles (normal radii 1 +- 0.1 clipped to [0.5, 1.5],
+    phi 0.9, FP64), 1000 accepted steps from creation, on the GPU and in the
+    FP64 restatement on the host cores (all threads, and one), each the
+    median of 3 runs; the same seed on both, with the end states compared."""
+    import oracle as ora
+
+    cfg = ca.Config.workload("foam2d_1e4")
+    steps = args.foam2d_steps
+    gpu, end = [], None
+    for rep in range(3):
+        sim = ca.Simulator(cfg)
+        b0 = sim.scalars()["bubble_steps"]
+        t0 = time.perf_counter()
+        sim.run_steps(steps, ip)
+        el = time.perf_counter() - t0
+        sc = sim.scalars()
+        gpu.append((sc["bubble_steps"] - b0) / el)
+        if end is None:
+            st, _ = sim.get_state()
+            end = {"count": sc["count"], "mean_r": float(st["x"][3].mean()), "time":
+                   sc["time_int"] + sc["time_frac"], "searches": sc["searches"],
+                   "attempts": sc["attempts"]}
+        sim.close()
+    info = cpu_info()
+    cpu = {}
+    oend = None
+    for threads in sorted({info["omp_threads"], 1}, reverse=True):
+        ora.set_threads(threads)
+        rates = []
+        for rep in range(3):
+            f = ora.Foam.synthetic(dim=cfg.dim, count=cfg.num_start, radius_dist=cfg.radius_dist,
+                                   r_mean=cfg.r_mean, r_std=cfg.r_std, r_lo=cfg.r_lo,
+                                   r_hi=cfg.r_hi, phi=cfg.phi, kappa=cfg.kappa,
+                                   k_param=cfg.k_param, mu=cfg.mu, sigma=cfg.sigma,
+                                   tol=cfg.error_tolerance, dt0=cfg.dt0, avg_rad=cfg.r_mean,
+                                   seed=cfg.seed)
+            b0 = f.scalars()["bubble_steps"]
+            el = f.run(steps)
+            sc = f.scalars()
+            rates.append((sc["bubble_steps"] - b0) / el)
+            if oend is None:
+                ost, _ = f.get_state()
+                oend = {"count": sc["count"], "mean_r": float(ost["x"][3].mean()),
+                        "time": sc["time_int"] + sc["time_frac"], "searches": sc["searches"],
+                        "attempts": sc["attempts"]}
+        cpu["threads_%d" % threads] = {"value": median(rates), "runs": rates, "threads": threads}
+    g = median(gpu)
+    log("foam2d_1e4: GPU %.4g, CPU %s bubble-steps/s" %
+        (g, {k: "%.4g" % v["value"] for k, v in cpu.items()}))
+    return {"workload": "2D periodic foam, 10,000 bubbles, normal radii (1, 0.1) clipped to "
+                        "[0.5, 1.5], phi 0.9, kappa 0.01, K 0.1, FP64, %d steps" % steps,
+            "gpu": {"value": g, "runs": gpu, "unit": "bubble-steps/s"},
+            "cpu_oracle": cpu, "cpu": info,
+            "gpu_over_cpu_all_threads": round(g / cpu["threads_%d" % info["omp_threads"]]["value"], 2),
+            "end_state": {"gpu": end, "oracle": oend,
+                          "mean_r_rel_diff": abs(end["mean_r"] / oend["mean_r"] - 1.0)}}
 
 
 def scaled_err(gpu, ref, scale):
@@ -263,17 +393,34 @@ def leg_mixed_error(ca, args, ip):
 
 
 def leg_serve(ca, args, ip):
+    """Poisson requests through the serving loop.  Offered load: the bubble-
+    steps the requests ask for (count x 500 steps, at most; a request ends
+    early once half its bubbles dissolved) per second of arrivals.  Queue
+    stability: the latency of the first and the last quarter of the requests
+    (a growing queue shows as a rising latency), and the time to drain after
+    the last arrival."""
+    import numpy as np
+
     reqs = ca.serving_requests(args.serve_requests, rate_per_s=50.0, precision=ca.PREC_MIXED)
     srv = ca.Server(hbm_budget_bytes=args.serve_budget_gib << 30, round_steps=args.serve_round,
                     precision=ca.PREC_MIXED)
-    for cfg, at in reqs:
-        srv.submit(cfg, max_steps=500, dissolve_fraction=0.5, arrival_s=at)
+    ids = [srv.submit(cfg, max_steps=500, dissolve_fraction=0.5, arrival_s=at) for cfg, at in reqs]
     st = srv.run()
+    res = [srv.result(i) for i in ids]
     srv.close()
     log("serve: %s" % json.dumps(st))
-    return {"requests": st["requests"], "arrival_rate_per_s": 50.0,
+    last = reqs[-1][1]
+    lat = np.array([r["finish_s"] - r["arrival_s"] for r in res])
+    q = max(1, len(res) // 4)
+    offered = sum(c.num_start * 500 for c, _ in reqs) / last
+    return {"requests": st["requests"], "arrival_rate_per_s": 50.0, "arrivals_s": round(last, 2),
             "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
             "throughput": st["throughput"], "unit": "bubble-steps/s", "wall_s": st["wall_s"],
+            "offered_max_bubble_steps_per_s": offered,
+            "served_bubble_steps_per_s_of_arrivals": st["bubble_steps"] / last,
+            "latency_first_quarter_p50_s": float(np.median(lat[:q])),
+            "latency_last_quarter_p50_s": float(np.median(lat[-q:])),
+            "drain_after_last_arrival_s": round(st["wall_s"] - last, 3),
             "max_resident": st["max_resident"], "peak_hbm_bytes": st["peak_bytes"],
             "hbm_budget_bytes": args.serve_budget_gib << 30, "round_steps": args.serve_round,
             "precision": "mixed"}
@@ -367,7 +514,7 @@ def measure_traffic(kernels=ATTEMPT_KERNELS):
         shutil.rmtree(folder, ignore_errors=True)
         copy = [v for k, v in vals.items() if "copy" in k.lower()]
         missing = [k for k in kernels if not vals.get(k)]
-        if r.returncode not in (0, -11, 139) and missing:
+        if r.returncode != 0:
             return None, "rocprofv3 %s pass failed (rc %d)" % (counter, r.returncode)
         if not copy or missing:
             return None, "%s pass: no reading for %s or the copy kernel" % (counter, missing)
@@ -408,6 +555,7 @@ def cpu_baseline(cfgs, steps, budget_s):
             partial = n
             break
     return {"value": work / elapsed, "unit": "bubble-steps/s", "cores": threads, "kind": "port",
+            "cpu": cpu_info(),
             "sample": "first %d foams of the 256-foam workload, %d steps each%s, FP64 OpenMP "
                       "restatement, %.1f s" % (done, steps, " (+%d steps of the next)" % partial
                                                if partial else "", elapsed)}
@@ -418,8 +566,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed steps of the batched workload")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--legs", default="batch,plain,foam,serve,mixed-error")
+    ap.add_argument("--legs", default="batch,plain,foam,foam2d,serve,mixed-error")
     ap.add_argument("--mixed-steps", type=int, default=500)
+    ap.add_argument("--foam2d-steps", type=int, default=1000)
     ap.add_argument("--foam-steps", type=int, default=2000)
     # trials long enough to see the search costs of a coarsening foam: at 64
     # steps the tuner preferred 2 bubbles per thread, which is about 1%
@@ -429,7 +578,7 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes bubbles_per_thread,search_tile,graph_steps "
                          "instead of autotuning")
-    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
     ap.add_argument("--serve-round", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -492,6 +641,8 @@ def main():
             out["speedup_vs_plain_fp64"] = round(out["value"] / pl["value"], 3)
     if "foam" in legs:
         out["foam3d_1e6_fp64"] = leg_foam(ca, args, ip)
+    if "foam2d" in legs:
+        out["foam2d_1e4_fp64_gpu_vs_cpu"] = leg_foam2d(ca, args, ip)
     if "serve" in legs:
         out["serving"] = leg_serve(ca, args, ip)
     if "mixed-error" in legs:

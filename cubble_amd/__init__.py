@@ -124,6 +124,17 @@ class _KernelTimes(ctypes.Structure):
                 ("correct_ms", ctypes.c_double), ("reps", ctypes.c_int32)]
 
 
+class _SearchTimes(ctypes.Structure):
+    _fields_ = [("search_ms", ctypes.c_double), ("lists_ms", ctypes.c_double),
+                ("list_kernel_ms", ctypes.c_double), ("reorder_ms", ctypes.c_double),
+                ("idle_ms", ctypes.c_double), ("bubbles", ctypes.c_double),
+                ("pairs", ctypes.c_double), ("reps", ctypes.c_int32)]
+
+
+def _search_times(t):
+    return {name: getattr(t, name) for name, _ in _SearchTimes._fields_}
+
+
 class _Tuning(ctypes.Structure):
     _fields_ = [("bubbles_per_thread", ctypes.c_int32), ("search_tile", ctypes.c_int32),
                 ("graph_steps", ctypes.c_int32)]
@@ -174,6 +185,7 @@ EXPORTS = [
     "cubble_create_tuned", "cubble_batch_create", "cubble_batch_size", "cubble_batch_integrate",
     "cubble_batch_run_steps", "cubble_batch_get_scalars", "cubble_batch_get_state",
     "cubble_batch_last_error", "cubble_batch_destroy", "cubble_batch_time_kernels",
+    "cubble_time_search", "cubble_batch_time_search",
     "cubble_server_create", "cubble_server_submit", "cubble_server_run", "cubble_server_result",
     "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
     "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
@@ -238,6 +250,8 @@ def lib():
         L.cubble_version.restype = ctypes.c_char_p
         L.cubble_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
                                           P(_KernelTimes)]
+        L.cubble_time_search.argtypes = [vp, ctypes.c_int32, P(_SearchTimes)]
+        L.cubble_batch_time_search.argtypes = [vp, ctypes.c_int32, P(_SearchTimes)]
         L.cubble_observe.argtypes = [vp, P(_Observables), _dp]
         L.cubble_set_stop_rules.argtypes = [vp, P(_StopRules)]
         L.cubble_reset_print_window.argtypes = [vp]
@@ -501,6 +515,13 @@ class Simulator:
         return {"predict_ms": t.predict_ms, "pair_ms": t.pair_ms, "correct_ms": t.correct_ms,
                 "reps": t.reps}
 
+    def time_search(self, reps=10):
+        """Average device ms of a search of the foam and of its parts, and of an
+        attempt group's idle removal and search kernels (the state is put back)."""
+        t = _SearchTimes()
+        self._call(lib().cubble_time_search(self._h, int(reps), ctypes.byref(t)))
+        return _search_times(t)
+
     def observe(self, energy=False):
         """Total energy and volume, box volume and the means of r, path and
         the velocities, reduced on the device; with energy=True also each
@@ -702,6 +723,11 @@ class Batch:
                                                    int(reps), ctypes.byref(t)))
         return {"predict_ms": t.predict_ms, "pair_ms": t.pair_ms, "correct_ms": t.correct_ms,
                 "reps": t.reps}
+
+    def time_search(self, reps=10):
+        t = _SearchTimes()
+        self._call(lib().cubble_batch_time_search(self._h, int(reps), ctypes.byref(t)))
+        return _search_times(t)
 
 
 class Server:

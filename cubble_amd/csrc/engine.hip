@@ -530,18 +530,35 @@ int Engine::read_header(int k, Foam &f) {
 
 int Engine::grow_lists(int k, int need, bool rebuild, bool keep) {
     Foam f;
-    int rc = read_header(k, f);
+    int rc = read_header(k, f);  // synchronises: no kernel uses the lists now
     if (rc) return rc;
     FoamRec &r = foams[k];
     const int ncap = std::max(need + need / 4 + 4, f.nbr_cap * 2);
-    void *nb = arena.alloc((size_t)ncap * f.cap * sizeof(int32_t));
-    if (!nb) return fail(CUBBLE_E_NOMEM, "no device memory to grow the neighbour lists");
-    // column q of bubble i is nbr[q * cap + i]: the old lists are the first
-    // nbr_cap columns of the grown block
-    if (keep)
-        E_TRY(hipMemcpyAsync(nb, r.nbr_block, (size_t)f.nbr_cap * f.cap * sizeof(int32_t),
-                             hipMemcpyDeviceToDevice, stream));
-    arena.release(r.nbr_block);
+    const size_t old_bytes = (size_t)f.nbr_cap * f.cap * sizeof(int32_t);
+    void *nb = nullptr;
+    if (keep) {
+        nb = arena.alloc((size_t)ncap * f.cap * sizeof(int32_t));
+        if (!nb) return fail(CUBBLE_E_NOMEM, "no device memory to grow the neighbour lists");
+        // column q of bubble i is nbr[q * cap + i]: the old lists are the
+        // first nbr_cap columns of the grown block
+        E_TRY(hipMemcpyAsync(nb, r.nbr_block, old_bytes, hipMemcpyDeviceToDevice, stream));
+        arena.release(r.nbr_block);
+    } else {
+        // the contents are rebuilt or replaced: free first, so that only the
+        // new block has to fit
+        arena.release(r.nbr_block);
+        nb = arena.alloc((size_t)ncap * f.cap * sizeof(int32_t));
+        if (!nb) {
+            // the old block's room is free again: take a block of the old
+            // size back (first fit: maybe elsewhere), so the foam stays whole
+            // (halted by its overflow) for the caller to decide
+            r.nbr_block = arena.alloc(old_bytes);
+            int32_t *p = static_cast<int32_t *>(r.nbr_block);
+            r.h.s.nbr = p;
+            E_TRY(hipMemcpy(&dfoams_[k].s.nbr, &p, sizeof(p), hipMemcpyHostToDevice));
+            return fail(CUBBLE_E_NOMEM, "no device memory to grow the neighbour lists");
+        }
+    }
     r.nbr_block = nb;
     int32_t *p = static_cast<int32_t *>(nb);
     r.h.s.nbr = p;
@@ -562,14 +579,34 @@ int Engine::grow_lists(int k, int need, bool rebuild, bool keep) {
     return write_dyn(k, f.d);
 }
 
+int Engine::fail_foam(int k, int code) {
+    foams[k].failed = code;
+    Foam f;
+    int rc = read_header(k, f);
+    if (rc) return rc;
+    f.d.failed = 1;
+    f.d.halt = 1;
+    f.d.overflow = 0;
+    f.d.need_search = 0;
+    f.d.need_remove = 0;
+    err.clear();
+    return write_dyn(k, f.d);
+}
+
 int Engine::settle(std::vector<Dyn> &d) {
     int rc = read_dyn_all(d);
     if (rc) return rc;
     bool grew = false;
     for (int k = 0; k < (int)d.size(); ++k) {
-        if (!d[k].overflow) continue;
+        if (!d[k].overflow || foams[k].failed) continue;
         // the step was committed; other foams of the batch are unaffected
-        if ((rc = grow_lists(k, d[k].max_nn, true))) return rc;
+        rc = grow_lists(k, d[k].max_nn, true);
+        if (CUBBLE_E_NOMEM == rc && isolate_failures) {
+            if ((rc = fail_foam(k, CUBBLE_E_NOMEM))) return rc;
+            grew = true;
+            continue;
+        }
+        if (rc) return rc;
         Foam f;
         if ((rc = read_header(k, f))) return rc;
         f.d.halt = (stop_reached(f.d) || (f.d.count_stop > 0 && f.d.count <= f.d.count_stop)) ? 1 : 0;
@@ -580,8 +617,15 @@ int Engine::settle(std::vector<Dyn> &d) {
     // capacity: an overflow halts the foam until the host settles
     for (int k = 0; k < (int)d.size(); ++k) {
         const int cap = foams[k].h.nbr_cap;
-        if (d[k].overflow || d[k].max_nn + std::max(4, cap / 8) <= cap) continue;
-        if ((rc = grow_lists(k, d[k].max_nn, false, true))) return rc;
+        if (d[k].overflow || foams[k].failed || d[k].max_nn + std::max(4, cap / 8) <= cap)
+            continue;
+        rc = grow_lists(k, d[k].max_nn, false, true);
+        // optional: without the memory the foam grows when it overflows
+        if (CUBBLE_E_NOMEM == rc) {
+            err.clear();
+            continue;
+        }
+        if (rc) return rc;
     }
     if (grew) rc = read_dyn_all(d);
     return rc;

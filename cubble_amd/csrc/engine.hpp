@@ -72,6 +72,7 @@ struct FoamRec {
     void *nbr_block = nullptr;
     int32_t num_end = 0;
     int64_t max_steps = 0;
+    int failed = 0;           // status code once the foam was failed alone
 };
 
 // Bytes of device memory one foam needs (the admission footprint).
@@ -146,6 +147,13 @@ class Engine {
     int precision = CUBBLE_PREC_FP64;
     std::string err;
     DeviceArena arena;
+    // With a fixed arena (the serving loop), a foam whose neighbour lists
+    // cannot grow is failed alone (FoamRec::failed, halted on the device)
+    // instead of failing the whole run.
+    bool isolate_failures = false;
+
+    // Mark foam k failed with `code` and halt it for good (isolate_failures).
+    int fail_foam(int k, int code);
 
   private:
     int grow_lists(int k, int need, bool rebuild, bool keep = false);

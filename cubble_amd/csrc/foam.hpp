@@ -109,6 +109,8 @@ struct Dyn {
     int32_t num_pairs;
     int32_t max_nn;
     int32_t count_stop;  // halt once count <= count_stop after a removal (0 = off)
+    int32_t failed;      // the host could not give the foam what it needs (list
+                         // growth out of memory): halted for good
     double dt;
     uint64_t time_int;
     double time_frac;

@@ -39,8 +39,12 @@ void launch_remove(const Grid &g);
 // rank-and-scatter reorder, neighbour lists.  Runs only for foams whose
 // need_search flag is set.
 void launch_search(const Grid &g);
-// Only the neighbour-list kernel (after the list capacity was raised).
+// Only the neighbour-list kernels (after the list capacity was raised).
 void launch_search_lists(const Grid &g);
+// One part of the search, for timing: the reorder, the list kernel alone, or
+// the whole list build (zero, lists, lower entries, sort) without totals.
+enum SearchPart { kPartLists = 0, kPartListKernel = 1, kPartReorder = 2 };
+void launch_search_part(const Grid &g, int part);
 
 // Set / clear the phase flags of every foam in the launch.
 void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear_halt);

@@ -161,6 +161,79 @@ int time_kernels(Engine &e, const cubble_integrate_params *ip, int reps, cubble_
     return CUBBLE_OK;
 }
 
+// The search phase timed from a copy of the device state, which is put back
+// afterwards (a search reorders the bubbles and resets the saved positions).
+// Each figure is the average over `reps` back-to-back launches between one
+// pair of events on the engine's stream.
+int time_search(Engine &e, int reps, cubble_search_times *out) {
+    if (!out || reps < 1) return e.fail(CUBBLE_E_INVALID, "bad argument");
+    int rc = CUBBLE_OK;
+    Snapshot snap;
+    if ((rc = e.snapshot(snap))) return rc;
+    const Grid g = e.grid();
+    std::vector<Dyn> d;
+    if ((rc = e.read_dyn_all(d))) {
+        e.free_snapshot(snap);
+        return rc;
+    }
+    double bubbles = 0.0;
+    for (const Dyn &x : d) bubbles += x.count;
+    hipEvent_t ev[2];
+    for (auto &v : ev) R_TRY(e, hipEventCreate(&v));
+    auto timed = [&](auto &&body, double &ms_out) -> int {
+        R_TRY(e, hipEventRecord(ev[0], e.stream));
+        for (int r = 0; r < reps; ++r) body();
+        R_TRY(e, hipEventRecord(ev[1], e.stream));
+        R_TRY(e, hipStreamSynchronize(e.stream));
+        float ms = 0.f;
+        R_TRY(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
+        ms_out = ms / reps;
+        return CUBBLE_OK;
+    };
+    // a whole search of every foam: the flags are set between the searches
+    // (k_s_fin clears them); that tiny kernel is timed alone and taken off
+    double flags_ms = 0.0, all_ms = 0.0, lists_ms = 0.0, kernel_ms = 0.0, reorder_ms = 0.0,
+           idle_ms = 0.0;
+    rc = timed([&] { launch_set_flags(g, 1, 0, 1); }, flags_ms);
+    if (!rc) rc = timed([&] {
+        launch_set_flags(g, 1, 0, 1);
+        launch_search(g);
+    }, all_ms);
+    // the parts: flags set once, the parts do not clear them
+    if (!rc) {
+        launch_set_flags(g, 1, 0, 1);
+        rc = timed([&] { launch_search_part(g, kPartLists); }, lists_ms);
+    }
+    if (!rc) rc = timed([&] { launch_search_part(g, kPartListKernel); }, kernel_ms);
+    if (!rc) rc = timed([&] { launch_search_part(g, kPartReorder); }, reorder_ms);
+    // an attempt group's removal and search kernels when no foam needs them
+    if (!rc) {
+        launch_set_flags(g, 0, 0, 1);
+        rc = timed([&] {
+            launch_remove(g);
+            launch_search(g);
+        }, idle_ms);
+    }
+    for (auto &v : ev) (void)hipEventDestroy(v);
+    const int rr = e.restore(snap);
+    e.free_snapshot(snap);
+    if (rc) return rc;
+    if (rr) return rr;
+    std::vector<Dyn> after;
+    if ((rc = e.read_dyn_all(after))) return rc;
+    double pairs = 0.0;
+    for (const Dyn &x : after) pairs += x.num_pairs;
+    out->search_ms = std::max(0.0, all_ms - flags_ms);
+    out->lists_ms = lists_ms;
+    out->list_kernel_ms = kernel_ms;
+    out->reorder_ms = reorder_ms;
+    out->idle_ms = idle_ms;
+    out->bubbles = bubbles;
+    out->pairs = pairs;
+    out->reps = reps;
+    return CUBBLE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -466,6 +539,10 @@ int cubble_time_kernels(cubble_sim *s, const cubble_integrate_params *ip, int32_
     return s ? time_kernels(s->e, ip, reps, out) : CUBBLE_E_INVALID;
 }
 
+int cubble_time_search(cubble_sim *s, int32_t reps, cubble_search_times *out) {
+    return s ? time_search(s->e, reps, out) : CUBBLE_E_INVALID;
+}
+
 // ---------------------------------------------------------------- batches
 int cubble_batch_create(const cubble_config *cfgs, int32_t n, const cubble_tuning *tuning,
                         cubble_batch **out) {
@@ -521,6 +598,10 @@ int cubble_batch_get_state(cubble_batch *b, int32_t k, cubble_state_view *view) 
 int cubble_batch_time_kernels(cubble_batch *b, const cubble_integrate_params *ip, int32_t reps,
                               cubble_kernel_times *out) {
     return b ? time_kernels(b->e, ip, reps, out) : CUBBLE_E_INVALID;
+}
+
+int cubble_batch_time_search(cubble_batch *b, int32_t reps, cubble_search_times *out) {
+    return b ? time_search(b->e, reps, out) : CUBBLE_E_INVALID;
 }
 
 int cubble_batch_autotune(cubble_batch *b, const cubble_integrate_params *ip,

@@ -168,6 +168,7 @@ int cubble_server_create(const cubble_server_opts *opts, cubble_server **out) {
     s->opts = *opts;
     if (s->opts.round_steps <= 0) s->opts.round_steps = 50;
     s->e.precision = opts->precision;
+    s->e.isolate_failures = true;
     const int rc = s->e.init((size_t)opts->hbm_budget_bytes, false);
     if (rc) {
         delete s;
@@ -256,6 +257,18 @@ int cubble_server_run(cubble_server *s, cubble_server_stats *stats) {
         const double t = now_s(t0);
         for (int k = (int)s->resident.size() - 1; k >= 0; --k) {
             Request &q = s->reqs[s->resident[k]];
+            if (e.foams[k].failed) {
+                // its lists could not grow in the budget: it fails alone
+                q.r.status = e.foams[k].failed;
+                q.r.finish_s = t;
+                q.r.final_count = d[k].count;
+                q.r.steps = d[k].steps;
+                q.r.bubble_steps = d[k].bubble_steps;
+                if ((rc = e.drop_foam(k))) return rc;
+                s->resident.erase(s->resident.begin() + k);
+                dropped = true;
+                continue;
+            }
             const bool done = d[k].steps >= q.max_steps ||
                               (q.count_stop > 0 && d[k].count <= q.count_stop);
             if (!done) continue;

@@ -192,6 +192,46 @@ int get_attempt(Engine &e, int k, cubble_attempt_view *v) {
     return CUBBLE_OK;
 }
 
+namespace {
+// CSR neighbour lists as the kernels need them: offsets from 0, non-
+// decreasing, within the view's buffer; entries in [0, n), never the bubble
+// itself, strictly ascending per list, and symmetric (j lists i whenever i
+// lists j); the flow's degrees, if given, not negative.
+int check_lists(Engine &e, const cubble_state_view *v, int n) {
+    const int32_t *off = v->nbr_off;
+    if (off[0] != 0) return e.fail(CUBBLE_E_INVALID, "set_state: nbr_off[0] must be 0");
+    for (int i = 0; i < n; ++i)
+        if (off[i + 1] < off[i])
+            return e.fail(CUBBLE_E_INVALID, "set_state: nbr_off must not decrease");
+    if (off[n] > 0 && (!v->nbr || (int64_t)off[n] > v->nbr_cap))
+        return e.fail(CUBBLE_E_INVALID, "set_state: nbr_off[count] exceeds the nbr buffer");
+    for (int i = 0; i < n; ++i) {
+        for (int32_t q = off[i]; q < off[i + 1]; ++q) {
+            const int32_t j = v->nbr[q];
+            if (j < 0 || j >= n || j == i)
+                return e.fail(CUBBLE_E_INVALID, "set_state: list entry " + std::to_string(j) +
+                                                    " of bubble " + std::to_string(i) +
+                                                    " is out of range or the bubble itself");
+            if (q > off[i] && j <= v->nbr[q - 1])
+                return e.fail(CUBBLE_E_INVALID, "set_state: list of bubble " + std::to_string(i) +
+                                                    " is not strictly ascending");
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int32_t q = off[i]; q < off[i + 1]; ++q) {
+            const int32_t j = v->nbr[q];
+            if (!std::binary_search(v->nbr + off[j], v->nbr + off[j + 1], i))
+                return e.fail(CUBBLE_E_INVALID, "set_state: lists are not symmetric (" +
+                                                    std::to_string(i) + " lists " +
+                                                    std::to_string(j) + ")");
+        }
+    if (v->degree)
+        for (int i = 0; i < n; ++i)
+            if (v->degree[i] < 0) return e.fail(CUBBLE_E_INVALID, "set_state: negative degree");
+    return CUBBLE_OK;
+}
+}  // namespace
+
 int set_state(Engine &e, int k, const cubble_state_view *v, const cubble_scalars *sc) {
     if (!v || !sc || k < 0 || k >= (int)e.foams.size())
         return e.fail(CUBBLE_E_INVALID, "bad argument");
@@ -200,6 +240,9 @@ int set_state(Engine &e, int k, const cubble_state_view *v, const cubble_scalars
     if (rc) return rc;
     const int n = sc->count;
     if (n < 0 || n > f.cap) return e.fail(CUBBLE_E_CAPACITY, "set_state: count exceeds capacity");
+    // the kernels index with the list entries: check every list before any
+    // upload, so a rejected view leaves the state untouched
+    if (v->nbr_off && (rc = check_lists(e, v, n))) return rc;
     const size_t db = (size_t)n * sizeof(double), ib = (size_t)n * sizeof(int32_t);
     for (int q = 0; q < 4; ++q) {
         if (2 == q && 2 == f.c.dim) continue;

@@ -454,6 +454,27 @@ int cubble_time_kernels(cubble_sim *sim, const cubble_integrate_params *ip, int3
 int cubble_batch_time_kernels(cubble_batch *b, const cubble_integrate_params *ip, int32_t reps,
                               cubble_kernel_times *out);
 
+/* The search phase (searchNeighbors, src/simulator.cu:60-178), timed from a
+ * copy of the device state that is put back afterwards, each figure the
+ * average of `reps` back-to-back launches between one pair of HIP events:
+ * a whole search of every foam; the list build alone (lists of the
+ * reference's neighborSearch / sortPairs, src/kernels.cu:617-719); the list
+ * kernel alone; the reorder (reorganizeByIndex, :721-770); and one attempt
+ * group's removal and search kernels when no foam needs them (they return
+ * at once).  No counterpart in the reference (it has NVTX ranges only). */
+typedef struct cubble_search_times {
+    double search_ms;
+    double lists_ms;
+    double list_kernel_ms;
+    double reorder_ms;
+    double idle_ms;
+    double bubbles;           /* bubbles searched (every foam)                */
+    double pairs;             /* half-list pairs the search finds              */
+    int32_t reps;
+} cubble_search_times;
+int cubble_time_search(cubble_sim *sim, int32_t reps, cubble_search_times *out);
+int cubble_batch_time_search(cubble_batch *b, int32_t reps, cubble_search_times *out);
+
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 
 /* Synthetic initial foam on the host: the exact arrays cubble_create uploads

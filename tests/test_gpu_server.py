@@ -102,3 +102,28 @@ def test_server_dissolution_stop():
         ws, _ = alone(cfg, r["steps"])
         assert np.array_equal(gs["x"], ws["x"]) and np.array_equal(gs["idx"], ws["idx"])
     srv.close()
+
+
+def test_server_fails_only_the_request_whose_lists_cannot_grow():
+    """A request with a tiny neighbour capacity overflows at its first
+    search; the budget holds both foams but not its grown lists, so it fails
+    alone (CUBBLE_E_NOMEM) and the other request finishes as if alone."""
+    a = config(3, 3000, phi=0.85, seed=61)
+    b = config(3, 3000, phi=0.85, seed=62, nbr_cap=2)
+    fa, fb = ca.foam_footprint(a), ca.foam_footprint(b)
+    print(f"server nomem: footprints {fa} {fb}", flush=True)
+    # B's lists grow from 2 to ~25 columns (+~90 KB per 1000 bubbles), more
+    # than the 15% of B's footprint left free
+    srv = ca.Server(hbm_budget_bytes=int(fa + 1.15 * fb), round_steps=10, keep_states=True)
+    ra = srv.submit(a, max_steps=30, arrival_s=0.0)
+    rb = srv.submit(b, max_steps=30, arrival_s=0.0)
+    stats = srv.run()
+    res_a, res_b = srv.result(ra), srv.result(rb)
+    assert res_b["status"] == ca.E_NOMEM and res_b["finish_s"] >= res_b["admit_s"] >= 0
+    assert res_a["status"] == 2 and res_a["steps"] == 30
+    assert stats["requests"] == 1
+    gs, gsc = srv.get_state(ra)
+    ws, wsc = alone(a, 30)
+    for key in gs:
+        assert np.array_equal(gs[key], ws[key]), key
+    srv.close()
