@@ -124,6 +124,22 @@ class _KernelTimes(ctypes.Structure):
                 ("correct_ms", ctypes.c_double), ("reps", ctypes.c_int32)]
 
 
+class Box(ctypes.Structure):
+    """cubble_box: dimensionality, walls (1 = wall), lower and upper corners."""
+    _fields_ = [("dim", ctypes.c_int32), ("wall", ctypes.c_int32 * 3),
+                ("lbb", ctypes.c_double * 3), ("tfr", ctypes.c_double * 3)]
+
+    @classmethod
+    def of(cls, dim, lbb, tfr, wall=(0, 0, 0)):
+        b = cls()
+        b.dim = dim
+        for k in range(3):
+            b.wall[k] = int(wall[k])
+            b.lbb[k] = float(lbb[k]) if k < len(lbb) else 0.0
+            b.tfr[k] = float(tfr[k]) if k < len(tfr) else 0.0
+        return b
+
+
 class _SearchTimes(ctypes.Structure):
     _fields_ = [("search_ms", ctypes.c_double), ("lists_ms", ctypes.c_double),
                 ("list_kernel_ms", ctypes.c_double), ("reorder_ms", ctypes.c_double),
@@ -185,7 +201,7 @@ EXPORTS = [
     "cubble_create_tuned", "cubble_batch_create", "cubble_batch_size", "cubble_batch_integrate",
     "cubble_batch_run_steps", "cubble_batch_get_scalars", "cubble_batch_get_state",
     "cubble_batch_last_error", "cubble_batch_destroy", "cubble_batch_time_kernels",
-    "cubble_time_search", "cubble_batch_time_search",
+    "cubble_time_search", "cubble_batch_time_search", "cubble_find_pairs",
     "cubble_server_create", "cubble_server_submit", "cubble_server_run", "cubble_server_result",
     "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
     "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
@@ -251,6 +267,8 @@ def lib():
         L.cubble_time_kernels.argtypes = [vp, P(_IntegrateParams), ctypes.c_int32,
                                           P(_KernelTimes)]
         L.cubble_time_search.argtypes = [vp, ctypes.c_int32, P(_SearchTimes)]
+        L.cubble_find_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, P(Box), ctypes.c_double,
+                                        vp, vp, ctypes.c_int64, vp, vp]
         L.cubble_batch_time_search.argtypes = [vp, ctypes.c_int32, P(_SearchTimes)]
         L.cubble_observe.argtypes = [vp, P(_Observables), _dp]
         L.cubble_set_stop_rules.argtypes = [vp, P(_StopRules)]
@@ -799,6 +817,16 @@ def half_pairs(off, nbr):
     p = np.stack([i[m], nbr[m]], axis=1)
     order = np.lexsort((p[:, 1], p[:, 0]))
     return p[order]
+
+
+def find_pairs(x, y, z, r, count, box, skin, pair_i, pair_j, pair_cap, num_pairs, stream=0):
+    """cubble_find_pairs on device addresses (ints): positions / radii x, y, z,
+    r (FP64), pair_i / pair_j (int32, pair_cap each), num_pairs (one int64);
+    stream a hipStream_t address (0 = the null stream).  Enqueues only."""
+    rc = lib().cubble_find_pairs(x, y, z, r, int(count), ctypes.byref(box), float(skin), pair_i,
+                                 pair_j, int(pair_cap), num_pairs, stream or None)
+    if rc != OK:
+        raise CubbleError(rc, lib().cubble_last_error(None).decode())
 
 
 def generate_foam(config):

@@ -159,6 +159,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int &total) {
 template <int D>
 __device__ __forceinline__ void wrapped_diff(const Consts &c, double x1, double y1, double z1,
                                              double x2, double y2, double z2, double d[3]) {
+#pragma clang fp contract(off)
     double d1[3] = {x1 - x2, y1 - y2, D == 3 ? z1 - z2 : 0.0};
     d[0] = d1[0];
     d[1] = d1[1];
@@ -347,6 +348,7 @@ __device__ __forceinline__ void add_flow(const Consts &c, const D4 &p, int deg, 
 
 // Squared length, x, y, z summed in order: reference src/vec.h:41-49.
 __device__ __forceinline__ double sq_len(const double d[3]) {
+#pragma clang fp contract(off)
     double t = 0;
     t += d[0] * d[0];
     t += d[1] * d[1];

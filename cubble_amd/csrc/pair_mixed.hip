@@ -18,10 +18,44 @@
 //     a = (R^2 - |d|^2) (|d|^2 - (r1 - r2)^2) / (4 |d|^2),
 // algebraically equal to the reference's r2^2 - ((r2^2 - r1^2 + |d|^2)/2)^2/|d|^2
 // (src/kernels.cu:240-249) but without its cancellation near tangency.
+//
+// The reference's overlap area jumps where one centre enters the other
+// bubble (|d| < max(r1, r2): the area becomes min(r1, r2)^2, :238-239), and
+// the imposed flow counts a larger neighbour only past the overlap test
+// (:281-307).  FP32 distances (relative error ~1e-7) would decide those two
+// branch points differently from FP64 for pairs that close to them, and one
+// such pair moves its bubbles' overlap and dR/dt by O(1).  Within a relative
+// band of 2^-16 around either threshold the kernel decides the branch from
+// the FP64 predictions, exactly as the FP64 path does; the band is ~15x the
+// FP32 error and holds about 1e-5 of the pairs.
 #include "dev_util.hpp"
 #include "launch.hpp"
 
 namespace cubble_amd {
+
+// Relative half-width of the band around a branch threshold that is decided
+// in FP64.
+constexpr float kBranchBand = 0x1p-16f;
+
+// The FP64 path's two branch decisions for pair (i, j), from the FP64
+// predictions (recomputed as k_correct does): bit 0, the pair overlaps
+// ((r1 + r2)^2 >= |d|^2); bit 1, a centre lies inside the other bubble
+// (|d|^2 < r1^2 or |d|^2 < r2^2), r1 the radius of the smaller index.
+template <int D, bool GAS>
+__device__ __attribute__((noinline)) int exact_branches(const Consts &c, const Roles &R,
+                                                        double h, int i, int j) {
+#pragma clang fp contract(off)
+    const D4 pi = predicted<D, GAS, float>(R, h, i);
+    const D4 pj = predicted<D, GAS, float>(R, h, j);
+    double d[3];
+    wrapped_diff<D>(c, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, d);
+    const double m = sq_len(d);
+    const double r1 = i < j ? pi.r : pj.r;
+    const double r2 = i < j ? pj.r : pi.r;
+    const double radii = r1 + r2;
+    const double r1sq = r1 * r1, r2sq = r2 * r2;
+    return (radii * radii >= m ? 1 : 0) | ((m < r1sq || m < r2sq) ? 2 : 0);
+}
 
 // FLOW: also the imposed-flow term (the neighbour-velocity sum in FP32, the
 // flow-box test on the bubble's FP64 prediction, recomputed here as the
@@ -57,9 +91,14 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
             const float m = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
             const float rad = ri_f + rj_f;
             const float over = rad * rad - m;
-            if (FLOW && (j < i || over >= 0.f)) {
+            if (FLOW) {
+                bool take = j < i || over >= 0.f;
+                if (j > i && fabsf(over) <= kBranchBand * (rad * rad))
+                    take = (exact_branches<D, GAS>(c, R, h, i, j) & 1) != 0;
+                if (take) {
 #pragma unroll
-                for (int a = 0; a < D; ++a) fv[a] += field<float>(R.v[a])[j];
+                    for (int a = 0; a < D; ++a) fv[a] += field<float>(R.v[a])[j];
+                }
             }
             if (over >= 0.f) {
                 const float rs = __builtin_amdgcn_rsqf(m);
@@ -71,7 +110,11 @@ __device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R
                     const float r2 = i < j ? rj_f : ri_f;
                     const float r1sq = r1 * r1, r2sq = r2 * r2;
                     float a2;
-                    if (m < r1sq || m < r2sq) {
+                    bool inside = m < r1sq || m < r2sq;
+                    if (fabsf(m - r1sq) <= kBranchBand * r1sq ||
+                        fabsf(m - r2sq) <= kBranchBand * r2sq)
+                        inside = (exact_branches<D, GAS>(c, R, h, i, j) & 2) != 0;
+                    if (inside) {
                         a2 = r1sq < r2sq ? r1sq : r2sq;
                     } else {
                         const float dl = r1 - r2;

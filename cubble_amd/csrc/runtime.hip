@@ -30,9 +30,14 @@ struct cubble_batch {
     Engine e;
 };
 
-namespace {
-
+namespace cubble_amd {
+// Message of the last failing call without a handle (create, stateless
+// calls), returned by cubble_last_error(NULL).
 std::string g_create_error;
+void set_global_error(const std::string &m) { g_create_error = m; }
+}  // namespace cubble_amd
+
+namespace {
 
 const char *kVersion = "cubble_amd 0.2 (gfx950)";
 

@@ -202,8 +202,9 @@ void cubble_config_default(cubble_config *cfg);
 /* Frees device memory and the stream (replaces end(), src/simulator.cu:514-525). */
 void cubble_destroy(cubble_sim *sim);
 
-/* Message of the last failing call on `sim` (or of the last failing create
- * when sim is NULL).  Valid until the next call on the same handle. */
+/* Message of the last failing call on `sim` (or, when sim is NULL, of the
+ * last failing create or call without a handle).  Valid until the next call
+ * on the same handle. */
 const char *cubble_last_error(const cubble_sim *sim);
 
 /* ---- the step interface --------------------------------------------------- */
@@ -474,6 +475,34 @@ typedef struct cubble_search_times {
 } cubble_search_times;
 int cubble_time_search(cubble_sim *sim, int32_t reps, cubble_search_times *out);
 int cubble_batch_time_search(cubble_batch *b, int32_t reps, cubble_search_times *out);
+
+/* ---- the pair search on caller-owned device arrays ---------------------- */
+
+/* A box: dimensionality, walls (1 = wall, 0 = periodic) and corners. */
+typedef struct cubble_box {
+    int32_t dim;
+    int32_t wall[3];
+    double lbb[3], tfr[3];
+} cubble_box;
+
+/* The pair output of the reference's searchNeighbors (src/simulator.cu:
+ * 60-178; neighborSearch / comparePair and sortPairs, src/kernels.cu:617-711)
+ * for bubbles the caller holds on the device: x, y, z (ignored and may be
+ * NULL in 2-D) and r, `count` FP64 values each, positions within one box
+ * length of the box and radii positive.  Writes every pair i < j with
+ * |wrap(x_i - x_j)|^2 < (r_i + r_j + skin)^2 -- the reference's Pairs {i, j},
+ * grouped by first index, each group ascending in j -- to the device arrays
+ * pair_i / pair_j, and the number of pairs to the device int64 *num_pairs.
+ * With more pairs than pair_cap, nothing is written beyond pair_cap: the
+ * groups that fit entirely are complete, the group at the boundary holds
+ * some of its pairs, and *num_pairs tells how much room a second call
+ * needs (the reference writes past its capacity, src/kernels.cu:630-632).  Enqueued on `stream` (a hipStream_t; NULL = the null stream);
+ * the host does not wait.  The scratch (about 40 bytes per bubble) is taken
+ * with hipMallocAsync and returned with hipFreeAsync on that stream.
+ * CUBBLE_E_INVALID for bad arguments; the message is cubble_last_error(NULL). */
+int cubble_find_pairs(const double *x, const double *y, const double *z, const double *r,
+                      int32_t count, const cubble_box *box, double skin, int32_t *pair_i,
+                      int32_t *pair_j, int64_t pair_cap, int64_t *num_pairs, void *stream);
 
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 
