@@ -24,6 +24,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "cubble_amd.h")
 
 OK = 0
 E_INVALID, E_HIP, E_NOMEM, E_IO, E_UNSUPPORTED, E_CAPACITY, E_STATE = -1, -2, -3, -4, -5, -6, -7
+FIND_PAIRS_NOALLOC_MAX = 800000  # CUBBLE_FIND_PAIRS_NOALLOC_MAX
 PREC_FP64, PREC_MIXED = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -202,6 +203,7 @@ EXPORTS = [
     "cubble_batch_run_steps", "cubble_batch_get_scalars", "cubble_batch_get_state",
     "cubble_batch_last_error", "cubble_batch_destroy", "cubble_batch_time_kernels",
     "cubble_time_search", "cubble_batch_time_search", "cubble_find_pairs",
+    "cubble_find_pairs_noalloc",
     "cubble_server_create", "cubble_server_submit", "cubble_server_run", "cubble_server_result",
     "cubble_server_get_state", "cubble_server_last_error", "cubble_server_destroy",
     "cubble_foam_footprint", "cubble_autotune", "cubble_retune", "cubble_batch_autotune",
@@ -269,6 +271,7 @@ def lib():
         L.cubble_time_search.argtypes = [vp, ctypes.c_int32, P(_SearchTimes)]
         L.cubble_find_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, P(Box), ctypes.c_double,
                                         vp, vp, ctypes.c_int64, vp, vp]
+        L.cubble_find_pairs_noalloc.argtypes = L.cubble_find_pairs.argtypes
         L.cubble_batch_time_search.argtypes = [vp, ctypes.c_int32, P(_SearchTimes)]
         L.cubble_observe.argtypes = [vp, P(_Observables), _dp]
         L.cubble_set_stop_rules.argtypes = [vp, P(_StopRules)]
@@ -819,12 +822,15 @@ def half_pairs(off, nbr):
     return p[order]
 
 
-def find_pairs(x, y, z, r, count, box, skin, pair_i, pair_j, pair_cap, num_pairs, stream=0):
-    """cubble_find_pairs on device addresses (ints): positions / radii x, y, z,
-    r (FP64), pair_i / pair_j (int32, pair_cap each), num_pairs (one int64);
-    stream a hipStream_t address (0 = the null stream).  Enqueues only."""
-    rc = lib().cubble_find_pairs(x, y, z, r, int(count), ctypes.byref(box), float(skin), pair_i,
-                                 pair_j, int(pair_cap), num_pairs, stream or None)
+def find_pairs(x, y, z, r, count, box, skin, pair_i, pair_j, pair_cap, num_pairs, stream=0,
+               noalloc=False):
+    """cubble_find_pairs (noalloc: cubble_find_pairs_noalloc) on device
+    addresses (ints): positions / radii x, y, z, r (FP64), pair_i / pair_j
+    (int32, pair_cap each), num_pairs (one int64); stream a hipStream_t
+    address (0 = the null stream).  Enqueues only."""
+    fn = lib().cubble_find_pairs_noalloc if noalloc else lib().cubble_find_pairs
+    rc = fn(x, y, z, r, int(count), ctypes.byref(box), float(skin), pair_i, pair_j, int(pair_cap),
+            num_pairs, stream or None)
     if rc != OK:
         raise CubbleError(rc, lib().cubble_last_error(None).decode())
 

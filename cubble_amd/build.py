@@ -24,7 +24,7 @@ ARCH = os.environ.get("CUBBLE_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["step_kernels.hip", "pair_mixed.hip", "search_kernels.hip", "remove_kernels.hip",
                "engine.hip", "state_io.hip", "runtime.hip", "server.hip", "autotune.hip",
-               "observe_kernels.hip", "pairs.hip"]
+               "observe_kernels.hip", "pairs.hip", "pairs_noalloc.hip"]
 HOST_SOURCES = ["host_init.cpp", "cli.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", INCLUDE, "-I", CSRC]

@@ -504,6 +504,23 @@ int cubble_find_pairs(const double *x, const double *y, const double *z, const d
                       int32_t count, const cubble_box *box, double skin, int32_t *pair_i,
                       int32_t *pair_j, int64_t pair_cap, int64_t *num_pairs, void *stream);
 
+/* cubble_find_pairs without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that must not call hipMalloc).  The
+ * scratch is a fixed device array of CUBBLE_FIND_PAIRS_NOALLOC_SCRATCH_MIB
+ * MiB that every device the library runs on holds until the process ends,
+ * enough for CUBBLE_FIND_PAIRS_NOALLOC_MAX bubbles; a larger count returns
+ * CUBBLE_E_CAPACITY before any work.  All calls on one device share that
+ * device's array: calls that overlap on one device (other streams, other
+ * host threads) race and must be serialised by the caller.  Same arguments,
+ * results and errors as cubble_find_pairs otherwise. */
+#define CUBBLE_FIND_PAIRS_NOALLOC_SCRATCH_MIB 32
+#define CUBBLE_FIND_PAIRS_NOALLOC_MAX 800000
+int cubble_find_pairs_noalloc(const double *x, const double *y, const double *z, const double *r,
+                              int32_t count, const cubble_box *box, double skin, int32_t *pair_i,
+                              int32_t *pair_j, int64_t pair_cap, int64_t *num_pairs,
+                              void *stream);
+
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 
 /* Synthetic initial foam on the host: the exact arrays cubble_create uploads
