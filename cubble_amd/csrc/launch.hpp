@@ -26,6 +26,9 @@ void launch_pair(const Grid &g, uint32_t flags);
 void launch_pair_mixed(const Grid &g, uint32_t flags);
 // S_A, S_Ar, S_OA, S_OAr from the predictor and pair partials (one block per foam)
 void launch_sums(const Grid &g);
+// The same after the mixed pair kernel, which first redoes the workgroups
+// that met a pair at a branch threshold with that branch decided in FP64.
+void launch_sums_mixed(const Grid &g, uint32_t flags);
 // corrector (mixed: the mixed-precision variant, for engines of that
 // precision), then (control) the dt controller / commit (one block per foam)
 void launch_correct(const Grid &g, uint32_t flags, bool mixed, bool control = true);
@@ -41,8 +44,8 @@ void launch_remove(const Grid &g);
 void launch_search(const Grid &g);
 // Only the neighbour-list kernels (after the list capacity was raised).
 void launch_search_lists(const Grid &g);
-// One part of the search, for timing: the reorder, the list kernel alone, or
-// the whole list build (zero, lists, lower entries, sort) without totals.
+// One part of the search, for timing: the list build (the list kernel alone)
+// or the reorder.
 enum SearchPart { kPartLists = 0, kPartListKernel = 1, kPartReorder = 2 };
 void launch_search_part(const Grid &g, int part);
 

@@ -24,10 +24,11 @@
 // the imposed flow counts a larger neighbour only past the overlap test
 // (:281-307).  FP32 distances (relative error ~1e-7) would decide those two
 // branch points differently from FP64 for pairs that close to them, and one
-// such pair moves its bubbles' overlap and dR/dt by O(1).  Within a relative
-// band of 2^-16 around either threshold the kernel decides the branch from
-// the FP64 predictions, exactly as the FP64 path does; the band is ~15x the
-// FP32 error and holds about 1e-5 of the pairs.
+// such pair moves its bubbles' overlap and dR/dt by O(1).  A bubble with a
+// pair within a relative band of 2^-16 around either threshold is summed a
+// second time with those branches decided from the FP64 predictions, exactly
+// as the FP64 path decides them; the band is ~15x the FP32 error and holds
+// about 1e-5 of the pairs.
 #include "dev_util.hpp"
 #include "launch.hpp"
 
@@ -38,14 +39,14 @@ namespace cubble_amd {
 constexpr float kBranchBand = 0x1p-16f;
 
 // The FP64 path's two branch decisions for pair (i, j), from the FP64
-// predictions (recomputed as k_correct does): bit 0, the pair overlaps
-// ((r1 + r2)^2 >= |d|^2); bit 1, a centre lies inside the other bubble
-// (|d|^2 < r1^2 or |d|^2 < r2^2), r1 the radius of the smaller index.
+// predictions (recomputed as k_correct does; pi is bubble i's): bit 0, the
+// pair overlaps ((r1 + r2)^2 >= |d|^2); bit 1, a centre lies inside the
+// other bubble (|d|^2 < r1^2 or |d|^2 < r2^2), r1 the radius of the smaller
+// index.
 template <int D, bool GAS>
-__device__ __attribute__((noinline)) int exact_branches(const Consts &c, const Roles &R,
-                                                        double h, int i, int j) {
+__device__ __forceinline__ int exact_branches(const Consts &c, const Roles &R, double h,
+                                              const D4 &pi, int i, int j) {
 #pragma clang fp contract(off)
-    const D4 pi = predicted<D, GAS, float>(R, h, i);
     const D4 pj = predicted<D, GAS, float>(R, h, j);
     double d[3];
     wrapped_diff<D>(c, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, d);
@@ -57,114 +58,180 @@ __device__ __attribute__((noinline)) int exact_branches(const Consts &c, const R
     return (radii * radii >= m ? 1 : 0) | ((m < r1sq || m < r2sq) ? 2 : 0);
 }
 
-// FLOW: also the imposed-flow term (the neighbour-velocity sum in FP32, the
-// flow-box test on the bubble's FP64 prediction, recomputed here as the
-// corrector does); same neighbour selection as the FP64 kernel.
-template <int D, bool GAS, bool FLOW>
-__device__ __forceinline__ void pair_range_mixed(const Consts &c, const Roles &R,
-                                                 const Scratch &S, size_t cap, int base, int ipt,
-                                                 int n, double h, double &soa, double &soar) {
-    const Q4 *__restrict__ q4 = S.q4;
-    const int *__restrict__ nnA = S.nn;
-    const int *__restrict__ nbr = S.nbr;
+struct MixedOut {
+    float acc[3], fv[3], ovl, dr;
+};
+
+// One bubble's pair sums over its list, ascending.  EXACT = false decides
+// every branch in FP32 and reports whether a pair came within the band of a
+// threshold; EXACT = true (run only for such bubbles) decides those pairs'
+// branches in FP64 instead, with everything else the same.
+template <int D, bool GAS, bool FLOW, bool EXACT>
+__device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &R,
+                                                  const Q4 *__restrict__ q4,
+                                                  const int *__restrict__ nbr, size_t cap,
+                                                  double h, const float step[3], int i,
+                                                  const Q4 &qi, int m_nn, MixedOut &o) {
     const float f0 = (float)c.f0;
     const float pi_f = (float)kPi;
+    const float ri_f = qi.r;
+    const float inv_ri = __builtin_amdgcn_rcpf(ri_f);
+    D4 pi{};
+    if (EXACT) pi = predicted<D, GAS, float>(R, h, i);
+    bool near = false;
+    for (int a = 0; a < 3; ++a) o.acc[a] = o.fv[a] = 0.f;
+    o.ovl = o.dr = 0.f;
+    auto body = [&](int j, const Q4 &qj) {
+        const float rj_f = qj.r;
+        // code differences are exact integers; one rounding to FP32 each
+        float dd[3] = {(float)fixed_diff(qi.x, qj.x) * step[0],
+                       (float)fixed_diff(qi.y, qj.y) * step[1],
+                       D == 3 ? (float)fixed_diff(qi.z, qj.z) * step[2] : 0.f};
+        const float m = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+        const float rad = ri_f + rj_f;
+        const float over = rad * rad - m;
+        if (FLOW) {
+            bool take = j < i || over >= 0.f;
+            if (j > i && fabsf(over) <= kBranchBand * (rad * rad)) {
+                near = true;
+                if (EXACT) take = (exact_branches<D, GAS>(c, R, h, pi, i, j) & 1) != 0;
+            }
+            if (take) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) o.fv[a] += field<float>(R.v[a])[j];
+            }
+        }
+        if (over >= 0.f) {
+            const float rs = __builtin_amdgcn_rsqf(m);
+            const float s = rs - __builtin_amdgcn_rcpf(rad);
+#pragma unroll
+            for (int a = 0; a < D; ++a) o.acc[a] += dd[a] * (f0 * s);
+            if (GAS) {
+                // a centre inside the other bubble: |d|^2 < r1^2 or r2^2, that
+                // is below the larger square; the product form is symmetric in
+                // the radii, so their order does not matter
+                const float sq_i = ri_f * ri_f, sq_j = rj_f * rj_f;
+                const float big = fmaxf(sq_i, sq_j);
+                float a2;
+                bool inside = m < big;
+                if (fabsf(m - big) <= kBranchBand * big) {
+                    near = true;
+                    if (EXACT) inside = (exact_branches<D, GAS>(c, R, h, pi, i, j) & 2) != 0;
+                }
+                if (inside) {
+                    a2 = fminf(sq_i, sq_j);
+                } else {
+                    const float dl = ri_f - rj_f;
+                    a2 = over * (m - dl * dl) * (0.25f * rs * rs);
+                    a2 = a2 < 0.f ? -a2 : a2;
+                }
+                const float oa = 3 == D ? a2 * pi_f : 2.0f * __builtin_amdgcn_sqrtf(a2);
+                o.ovl += oa;
+                o.dr += oa * (__builtin_amdgcn_rcpf(rj_f) - inv_ri);
+            }
+        }
+    };
+    if (EXACT) {
+        // the rare pass: one neighbour at a time, so that its FP64 work does
+        // not add to the registers of the pipelined pass
+        const auto ids = gmem(nbr);
+        const auto recs = gmem(q4);
+        for (int q = 0; q < m_nn; ++q) {
+            const int j = ids[(size_t)q * cap + i];
+            body(j, load_rec(recs + j));
+        }
+    } else {
+        for_neighbours(nbr, q4, cap, i, m_nn, body);
+    }
+    return near;
+}
+
+// One bubble's pair terms, stored (velocities, dR/dt, overlap); returns its
+// overlap sum and radius for the block partials, and whether a pair came
+// within the band of a branch threshold.  FLOW: also the imposed-flow term
+// (the neighbour-velocity sum in FP32, the flow-box test on the bubble's FP64
+// prediction, recomputed here as the corrector does); same neighbour
+// selection as the FP64 kernel.
+template <int D, bool GAS, bool FLOW, bool EXACT>
+__device__ __forceinline__ float mixed_bubble(const Consts &c, const Roles &R, const Scratch &S,
+                                              size_t cap, double h, const float step[3], int i,
+                                              float &ri, bool &near) {
+    const Q4 qi = load_rec(gmem(S.q4) + i);
+    MixedOut o;
+    near = pair_bubble_mixed<D, GAS, FLOW, EXACT>(c, R, S.q4, S.nbr, cap, h, step, i, qi,
+                                                  S.nn[i], o);
+    if (FLOW) add_flow<D>(c, predicted<D, GAS, float>(R, h, i), S.deg[i], o.fv, o.acc);
+#pragma unroll
+    for (int a = 0; a < D; ++a) stream_store(field<float>(R.vp[a]) + i, o.acc[a]);
+    stream_store(field<float>(R.vp[3]) + i, o.dr);
+    stream_store(field<float>(S.ov) + i, o.ovl);
+    ri = qi.r;
+    return o.ovl;
+}
+
+// The bubbles of bubble-space block lb (thread t takes base + k kBlock + t,
+// k < ipt): pair terms and the block's overlap sums, per thread in k order.
+template <int D, bool GAS, bool FLOW, bool EXACT>
+__device__ __forceinline__ void mixed_block(const Consts &c, const Roles &R, const Scratch &S,
+                                            size_t cap, int base, int ipt, int n, double h,
+                                            double &soa, double &soar, bool &near_any) {
     float step[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) step[a] = fixed_step(c.interval[a], c.wall[a] != 0);
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        const Q4 qi = load_rec(gmem(q4) + i);
-        const float ri_f = qi.r;
-        const float inv_ri = __builtin_amdgcn_rcpf(ri_f);
-        float acc[3] = {0.f, 0.f, 0.f};
-        float fv[3] = {0.f, 0.f, 0.f};
-        float ovl = 0.f, dr = 0.f;
-        const int m_nn = nnA[i];
-        for_neighbours(nbr, q4, cap, i, m_nn, [&](int j, const Q4 &qj) {
-            const float rj_f = qj.r;
-            // code differences are exact integers; one rounding to FP32 each
-            float dd[3] = {(float)fixed_diff(qi.x, qj.x) * step[0],
-                           (float)fixed_diff(qi.y, qj.y) * step[1],
-                           D == 3 ? (float)fixed_diff(qi.z, qj.z) * step[2] : 0.f};
-            const float m = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
-            const float rad = ri_f + rj_f;
-            const float over = rad * rad - m;
-            if (FLOW) {
-                bool take = j < i || over >= 0.f;
-                if (j > i && fabsf(over) <= kBranchBand * (rad * rad))
-                    take = (exact_branches<D, GAS>(c, R, h, i, j) & 1) != 0;
-                if (take) {
-#pragma unroll
-                    for (int a = 0; a < D; ++a) fv[a] += field<float>(R.v[a])[j];
-                }
-            }
-            if (over >= 0.f) {
-                const float rs = __builtin_amdgcn_rsqf(m);
-                const float s = rs - __builtin_amdgcn_rcpf(rad);
-#pragma unroll
-                for (int a = 0; a < D; ++a) acc[a] += dd[a] * (f0 * s);
-                if (GAS) {
-                    const float r1 = i < j ? ri_f : rj_f;
-                    const float r2 = i < j ? rj_f : ri_f;
-                    const float r1sq = r1 * r1, r2sq = r2 * r2;
-                    float a2;
-                    bool inside = m < r1sq || m < r2sq;
-                    if (fabsf(m - r1sq) <= kBranchBand * r1sq ||
-                        fabsf(m - r2sq) <= kBranchBand * r2sq)
-                        inside = (exact_branches<D, GAS>(c, R, h, i, j) & 2) != 0;
-                    if (inside) {
-                        a2 = r1sq < r2sq ? r1sq : r2sq;
-                    } else {
-                        const float dl = r1 - r2;
-                        a2 = over * (m - dl * dl) * (0.25f * rs * rs);
-                        a2 = a2 < 0.f ? -a2 : a2;
-                    }
-                    const float oa = 3 == D ? a2 * pi_f : 2.0f * __builtin_amdgcn_sqrtf(a2);
-                    ovl += oa;
-                    dr += oa * (__builtin_amdgcn_rcpf(rj_f) - inv_ri);
-                }
-            }
-        });
-        if (FLOW) add_flow<D>(c, predicted<D, GAS, float>(R, h, i), S.deg[i], fv, acc);
-#pragma unroll
-        for (int a = 0; a < D; ++a) stream_store(field<float>(R.vp[a]) + i, acc[a]);
-        stream_store(field<float>(R.vp[3]) + i, dr);
-        stream_store(field<float>(S.ov) + i, ovl);
+        float ri = 1.f;
+        bool near = false;
+        const float ovl = mixed_bubble<D, GAS, FLOW, EXACT>(c, R, S, cap, h, step, i, ri, near);
+        near_any = near_any || near;
         soa += (double)ovl;
-        soar += (double)ovl * (1.0 / (double)ri_f);
+        soar += (double)ovl * (1.0 / (double)ri);
     }
 }
 
-// One kernel per FLOW value (the flow variant needs more registers).
+template <bool FLOW, bool EXACT>
+__device__ __forceinline__ void mixed_block_dispatch(const Foam *F, int lb, uint32_t flags,
+                                                     double &soa, double &soar, bool &near_any) {
+    const int n = F->d.count, ipt = F->ipt;
+    const Consts c = F->c;
+    const Roles R = F->r;
+    const Scratch S = F->s;
+    const size_t cap = (size_t)F->cap;
+    const int base = lb * kBlock * ipt;
+    const double h = 0.5 * F->d.dt;
+    const bool gas = (flags & kGas) != 0;
+    if (3 == c.dim) {
+        if (gas) mixed_block<3, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        else mixed_block<3, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+    } else {
+        if (gas) mixed_block<2, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        else mixed_block<2, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+    }
+}
+
+// One kernel per FLOW value (the flow variant needs more registers).  Every
+// branch is decided in FP32; a workgroup that met a pair within the band of
+// a threshold marks itself (block partial slot 2) and counts in the foam's
+// n_near, and k_sums_mixed recomputes its bubbles with those branches
+// decided in FP64.
 template <bool FLOW>
 __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *bt,
                                                        uint32_t flags) {
     const Ctx cx = bubble_ctx(foams, bt);
     Foam *F = cx.F;
     if (F->d.halt) return;
-    const int n = F->d.count, ipt = F->ipt;
-    const Consts c = F->c;
-    const Roles R = F->r;
-    const Scratch S = F->s;
-    const size_t cap = (size_t)F->cap;
-    const int base = cx.lb * kBlock * ipt;
     double soa = 0.0, soar = 0.0;
-    const bool gas = (flags & kGas) != 0;
-    const double h = 0.5 * F->d.dt;
-    if (3 == c.dim) {
-        if (gas) pair_range_mixed<3, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
-        else pair_range_mixed<3, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
-    } else {
-        if (gas) pair_range_mixed<2, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
-        else pair_range_mixed<2, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
-    }
+    bool near = false;
+    mixed_block_dispatch<FLOW, false>(F, cx.lb, flags, soa, soar, near);
+    const int marked = __syncthreads_or(near ? 1 : 0);
     soa = block_sum(soa);
     soar = block_sum(soar);
     if (threadIdx.x == 0) {
-        S.part[cx.lb * kPartD + 2] = soa;
-        S.part[cx.lb * kPartD + 3] = soar;
+        F->s.part[cx.lb * kPartD + 2] = soa;
+        F->s.part[cx.lb * kPartD + 3] = soar;
+        F->s.part_i[cx.lb * kPartI + 2] = marked;
+        if (marked) atomicAdd(&F->d.n_near, 1);
     }
 }
 
@@ -175,6 +242,65 @@ void launch_pair_mixed(const Grid &g, uint32_t flags) {
     else
         hipLaunchKernelGGL(k_pair_mixed<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream,
                            g.foams, g.bt_bub, flags);
+}
+
+__device__ __forceinline__ double sum_parts(const double *part, int nb, int q) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) a += part[b * kPartD + q];
+    return block_sum(a);
+}
+
+// One block per foam, after k_pair_mixed: the workgroups it marked are
+// recomputed here with their threshold branches decided in FP64 -- the same
+// bubbles per thread, in the same order, so every stored value and the
+// block's overlap sums are those of a pass deciding them so from the start
+// -- then S_A, S_Ar, S_OA, S_OAr from the partials in a fixed order, as
+// k_sums does.
+__global__ __launch_bounds__(kBlock) void k_sums_mixed(Foam *foams, uint32_t flags) {
+    Foam *F = foams + blockIdx.x;
+    if (F->d.halt) return;
+    const int nb = F->nblk;
+    if (F->d.n_near > 0) {
+        __shared__ int list[kBlock];
+        for (int b0 = 0; b0 < nb; b0 += kBlock) {
+            const int b = b0 + (int)threadIdx.x;
+            const int mark = b < nb && F->s.part_i[b * kPartI + 2] ? 1 : 0;
+            int tot = 0;
+            const int pos = block_excl_scan(mark, tot);
+            if (mark) list[pos] = b;
+            __syncthreads();
+            for (int q = 0; q < tot; ++q) {
+                const int lb = list[q];
+                double soa = 0.0, soar = 0.0;
+                bool near = false;
+                if (flags & kFlow) mixed_block_dispatch<true, true>(F, lb, flags, soa, soar, near);
+                else mixed_block_dispatch<false, true>(F, lb, flags, soa, soar, near);
+                soa = block_sum(soa);
+                soar = block_sum(soar);
+                if (threadIdx.x == 0) {
+                    F->s.part[lb * kPartD + 2] = soa;
+                    F->s.part[lb * kPartD + 3] = soar;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const double *part = F->s.part;
+    const double sa = sum_parts(part, nb, 0);
+    const double sar = sum_parts(part, nb, 1);
+    const double s_oa = sum_parts(part, nb, 2);
+    const double s_oar = sum_parts(part, nb, 3);
+    if (threadIdx.x == 0) {
+        F->d.s_a = sa;
+        F->d.s_ar = sar;
+        F->d.s_oa = s_oa;
+        F->d.s_oar = s_oar;
+        F->d.n_near = 0;
+    }
+}
+
+void launch_sums_mixed(const Grid &g, uint32_t flags) {
+    hipLaunchKernelGGL(k_sums_mixed, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
 }
 
 }  // namespace cubble_amd

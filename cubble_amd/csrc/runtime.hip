@@ -136,7 +136,8 @@ int time_kernels(Engine &e, const cubble_integrate_params *ip, int reps, cubble_
     }
     R_TRY(e, hipEventRecord(x[2], e.stream));
     // the corrector alone (its inputs: the sums, reduced once before)
-    launch_sums(g);
+    if (CUBBLE_PREC_MIXED == e.precision) launch_sums_mixed(g, flags);
+    else launch_sums(g);
     R_TRY(e, hipEventRecord(x[3], e.stream));
     for (int r = 0; r < reps; ++r) launch_correct(g, flags, CUBBLE_PREC_MIXED == e.precision, false);
     R_TRY(e, hipEventRecord(x[4], e.stream));

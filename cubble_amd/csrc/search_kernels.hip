@@ -13,21 +13,13 @@
 //                 atomic order is arbitrary) and scatters its whole state to
 //                 its new slot in the alternate buffers; saved <- current
 //                 (reference reorganizeByIndex :721-770)
-//   k_s_search  : the upper list (j > i) of every bubble, ascending, over the
-//                 de-duplicated 3^D cell neighbourhood: each unordered pair
-//                 is tested once, by its smaller bubble, as the reference's
-//                 half-shell does (neighborSearch / comparePair
-//                 src/kernels.cu:617-701, getNeighborCellIndex :1116-1189,
-//                 simulator.cu:153-156); the list fills the bubble's columns
-//                 from the top down, and each hit counts one lower entry of
-//                 j.  Counts beyond the list capacity are reported, never
-//                 written out of bounds.
-//   k_s_lower   : every upper entry (i, j) puts i into j's lower list, from
-//                 the bottom of j's columns (reference sortPairs: pairs
-//                 grouped by bubble, :703-711)
-//   k_s_lists   : each bubble sorts its lower entries and moves its upper
-//                 list down after them: the full list, ascending; length,
-//                 degree (countNumNeighbors :713-719) and the capacity check
+//   k_s_search  : full neighbour list of every bubble, ascending, over the
+//                 de-duplicated 3^D cell neighbourhood, one wavefront per
+//                 cell (reference neighborSearch / comparePair
+//                 src/kernels.cu:617-701 with getNeighborCellIndex
+//                 :1116-1189); the reference's half list is the subset j > i.
+//                 Counts beyond the list capacity are reported, never written
+//                 out of bounds.
 //   k_s_fin     : one block per foam makes the reordered buffers current
 //                 (the pointer rotation of simulator.cu:96-147) and totals the
 //                 pair count
@@ -234,8 +226,6 @@ __device__ __forceinline__ void reorder_range(const Roles &R, const Scratch &S, 
             field<V>(R.vs[q])[np] = field<V>(R.vo[q])[i];
         }
         S.x4[np] = D4{xs[0], xs[1], xs[2], xs[3]};
-        S.del_list[np] = 0;  // lower-entry count and cursor of the list build
-        S.plan_src[np] = 0;
         R.path_new[np] = R.path[i];
         field<V>(R.err_alt)[np] = field<V>(R.err)[i];
 #pragma unroll
@@ -344,17 +334,15 @@ __device__ __forceinline__ XRuns x_runs(const AxisCells &X) {
     return r;
 }
 
-// Wave-uniform append of the ballot `m` (non-zero) of candidates j > i to
-// own bubble i's upper list.  Lane `ii` of the group (ii is wave-uniform)
-// keeps the count: it is read with readlane and updated on the scalar unit.
-// A hit lane's place is the count plus the hits in the lanes below it
-// (mbcnt); the hit itself is re-derived from `m`, so the comparison result
-// need not stay live across the branch.  The upper list fills i's columns
-// from the top down (place q in column nbr_cap - 1 - q), leaving the bottom
-// for the lower entries; each hit also counts one lower entry of j.
+// Wave-uniform append of the ballot `m` (non-zero) of candidates j to own
+// bubble i's list.  Lane `ii` of the group (ii is wave-uniform) keeps the
+// count: it is read with readlane and updated on the scalar unit.  A hit
+// lane's slot is the count plus the hits in the lanes below it (mbcnt); the
+// hit itself is re-derived from `m`, so the comparison result need not stay
+// live across the branch.
 __device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i, int j,
-                                            int *__restrict__ nbr_, int *__restrict__ low,
-                                            unsigned cap, int nbr_cap, int &cnt) {
+                                            int *__restrict__ nbr_, unsigned cap, int nbr_cap,
+                                            int &cnt) {
     const int lane = (int)(threadIdx.x & 63);
     const int base = __builtin_amdgcn_readlane(cnt, ii);
     const int upd = base + __popcll(m);  // scalar
@@ -364,23 +352,25 @@ __device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i,
             (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         const int pos = base + below;
         auto nbr = (__attribute__((address_space(1))) int *)(nbr_ + i);
-        const int col = nbr_cap - 1 - pos;
-        if (upd <= nbr_cap) nbr[(uint64_t)(unsigned)col * cap] = j;  // all fit (scalar test)
-        else if (pos < nbr_cap) nbr[(uint64_t)(unsigned)col * cap] = j;
-        atomicAdd(low + j, 1);
+        if (upd <= nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;  // all fit (scalar test)
+        else if (pos < nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;
     }
     cnt = lane == ii ? upd : cnt;
 }
 
-// The upper list's length goes to nn (uncapped), for k_s_lower / k_s_lists.
-__device__ __forceinline__ void finish_group(int cb, int nb, int cnt, int *__restrict__ nn) {
-    if ((int)(threadIdx.x & 63) < nb) nn[cb + (int)(threadIdx.x & 63)] = cnt;
-}
-
-// Lanes above lane k - 1 (k <= 0: all, k >= 64: none), built on the scalar
-// unit: the candidates of a chunk that come after a given place.
-__device__ __forceinline__ unsigned long long lanes_from(int k) {
-    return k <= 0 ? ~0ull : (k >= 64 ? 0ull : (~0ull << k));
+// The list length goes to nn and, as the degree the imposed flow divides by,
+// to deg (which a removal moves but does not reduce).
+__device__ __forceinline__ void finish_group(int cb, int nb, int cnt, int nbr_cap,
+                                             int *__restrict__ nn, int *__restrict__ deg,
+                                             int &sum_nn, int &max_nn, int &ovf) {
+    if ((int)(threadIdx.x & 63) < nb) {
+        const int v = cnt < nbr_cap ? cnt : nbr_cap;
+        nn[cb + (int)(threadIdx.x & 63)] = v;
+        deg[cb + (int)(threadIdx.x & 63)] = v;
+        sum_nn += cnt;
+        max_nn = max(max_nn, cnt);
+        if (cnt > nbr_cap) ovf = 1;
+    }
 }
 
 // Neighbour lists, one wavefront per cell, any box (the general path).
@@ -398,9 +388,10 @@ template <int D>
 __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *__restrict__ x4_,
                                                     const int *__restrict__ start_,
                                                     int *__restrict__ nbr, int *__restrict__ nn,
-                                                    int *__restrict__ low,
+                                                    int *__restrict__ deg,
                                                     const int cd[3], size_t cap, int nbr_cap,
-                                                    int cell, int i0, int i1) {
+                                                    int cell, int i0, int i1, int &sum_nn,
+                                                    int &max_nn, int &ovf) {
     const auto x4 = gmem(x4_);
     const auto x4s = cmem(x4_);
     const auto start = cmem(start_);
@@ -428,9 +419,7 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
                 for (int part = 0; part < 2; ++part) {
                     if (a >= Z.n || b >= Y.n || (1 == part && !xr.two)) continue;
                     const int row = Z.at(a) * cd[0] * cd[1] + Y.at(b) * cd[0];
-                    // only candidates above the group's first own bubble can
-                    // be above an own bubble (j > i: each pair once)
-                    const int j0 = max(start[row + (0 == part ? xr.a0 : xr.a1)], cb + 1);
+                    const int j0 = start[row + (0 == part ? xr.a0 : xr.a1)];
                     const int j1 = start[row + (0 == part ? xr.b0 : xr.b1) + 1];
                     for (int jb = j0; jb < j1; jb += 64) {
                         const int j = jb + lane;
@@ -446,18 +435,17 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
                             double d[3];
                             wrapped_diff<D>(c, px, py, pz, pj.x, pj.y, pj.z, d);
                             // one ballot per comparison: each is the comparison's own
-                            // lane mask, with no boolean materialised in a register;
-                            // candidates j > i are the lanes above i - jb
+                            // lane mask, with no boolean materialised in a register
                             const unsigned long long m =
-                                vmask & lanes_from(i - jb + 1) &
+                                vmask & __builtin_amdgcn_ballot_w64(j != i) &
                                 __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
-                            if (m) append_hits(m, ii, i, j, nbr, low, (unsigned)cap, nbr_cap, cnt);
+                            if (m) append_hits(m, ii, i, j, nbr, (unsigned)cap, nbr_cap, cnt);
                         }
                     }
                 }
             }
         }
-        finish_group(cb, nb, cnt, nn);
+        finish_group(cb, nb, cnt, nbr_cap, nn, deg, sum_nn, max_nn, ovf);
     }
 }
 
@@ -493,9 +481,10 @@ template <int D>
 __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restrict__ x4_,
                                             const int *__restrict__ start_,
                                             int *__restrict__ nbr, int *__restrict__ nn,
-                                            int *__restrict__ low,
+                                            int *__restrict__ deg,
                                             const int cd[3], size_t cap, int nbr_cap, int cell,
-                                            int i0, int i1, double rmax) {
+                                            int i0, int i1, double rmax, int &sum_nn,
+                                            int &max_nn, int &ovf) {
     const auto x4 = gmem(x4_);
     const auto x4s = cmem(x4_);
     const auto start = cmem(start_);
@@ -562,17 +551,6 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     for (int cb = i0; cb < i1; cb += 64) {
         const int nb = uniform_i(min(64, i1 - cb));
         int cnt = 0;
-        // place of own bubble cb in the ascending candidate sequence (the own
-        // cell lies in exactly one run): candidates before it are below every
-        // own bubble of the group and are never tested (j > i: each pair once)
-        int ps = total;
-#pragma unroll
-        for (int s = 0; s < kRuns; ++s) {
-            const int js = pre[s] + off[s];
-            const int je = (s + 1 < kRuns ? pre[s + 1] : total) + off[s];
-            if (js <= cb && cb < je) ps = cb - off[s];
-        }
-        ps = uniform_i(ps);
         // rows each own bubble (one per lane) can reach
         int reach_rows = (1 << kRows) - 1;
         if (prune) {
@@ -595,7 +573,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             }
         }
         const unsigned long long own_all = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
-        for (int p0 = ps; p0 < total; p0 += 64) {
+        for (int p0 = 0; p0 < total; p0 += 64) {
             // rows this chunk of candidates touches (scalar)
             int rows = 0;
 #pragma unroll
@@ -635,8 +613,6 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             // hit mask of own bubble i against the chunk (one ballot per
             // comparison: each is the comparison's own lane mask, with no
             // boolean materialised in a register)
-            // own bubble i sits at place ps + (i - cb); the candidates above
-            // it are the lanes from that place + 1 on (a scalar mask)
             auto test = [&](auto flip, int i) {
                 const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
                 const double pz = 3 == D ? x4s[i].z : 0.0;
@@ -647,7 +623,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                     for (int k = 0; k < D; ++k) d[k] = d[k] - sh[k];
                 }
                 const double maxd = pr + pj.r + skin;
-                return vmask & lanes_from(ps + (i - cb) - p0 + 1) &
+                return vmask & __builtin_amdgcn_ballot_w64(j != i) &
                        __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
             };
             // two own bubbles per iteration: half the loop and load overhead on
@@ -662,18 +638,18 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                         todo &= todo - 1ull;
                         const unsigned long long ma = test(flip, cb + ia);
                         const unsigned long long mb = test(flip, cb + ib);
-                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, low, (unsigned)cap, nbr_cap, cnt);
-                        if (mb) append_hits(mb, ib, cb + ib, j, nbr, low, (unsigned)cap, nbr_cap, cnt);
+                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
+                        if (mb) append_hits(mb, ib, cb + ib, j, nbr, (unsigned)cap, nbr_cap, cnt);
                     } else {
                         const unsigned long long ma = test(flip, cb + ia);
-                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, low, (unsigned)cap, nbr_cap, cnt);
+                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
                     }
                 }
             };
             if (anyflip) own_loop(std::true_type{});
             else own_loop(std::false_type{});
         }
-        finish_group(cb, nb, cnt, nn);
+        finish_group(cb, nb, cnt, nbr_cap, nn, deg, sum_nn, max_nn, ovf);
     }
 }
 
@@ -721,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
         end = min(block_lower_bound(S.cell_start, ncell, b1), ncell);
     }
     const size_t cap = (size_t)F->cap;
-    int *low = S.del_list;  // lower-entry counts (zeroed by the reorder)
+    int sum_nn = 0, max_nn = 0, ovf = 0;
     // the fast path needs 5 or more cells along every periodic axis
     bool fast = true;
 #pragma unroll
@@ -742,153 +718,20 @@ __global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt
             const int o1 = min(o0 + kOwnUnit, s1);
             if (3 == c.dim) {
                 if (fast)
-                    search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, low, cd, cap, F->nbr_cap,
-                                   cell, o0, o1, rmax);
+                    search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
+                                   F->nbr_cap, cell, o0, o1, rmax, sum_nn, max_nn, ovf);
                 else
-                    search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, low, cd, cap,
-                                           F->nbr_cap, cell, o0, o1);
+                    search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
+                                           F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
             } else {
                 if (fast)
-                    search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, low, cd, cap, F->nbr_cap,
-                                   cell, o0, o1, rmax);
+                    search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
+                                   F->nbr_cap, cell, o0, o1, rmax, sum_nn, max_nn, ovf);
                 else
-                    search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, low, cd, cap,
-                                           F->nbr_cap, cell, o0, o1);
+                    search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
+                                           F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
             }
         }
-    }
-}
-
-// Every upper entry j of bubble i (columns nbr_cap - 1, nbr_cap - 2, ...)
-// puts i into j's lower list, at the bottom of j's columns: the place comes
-// from j's cursor, in no particular order (k_s_lists sorts).  With the
-// counts of k_s_search, a list that fits keeps its lower and upper parts
-// apart; one that does not is rebuilt after the capacity grows, so here only
-// the bounds matter.
-__global__ __launch_bounds__(kBlock) void k_s_lower(Foam *foams, const int2 *bt) {
-    const Ctx cx = bubble_ctx(foams, bt);
-    Foam *F = cx.F;
-    if (!F->d.need_search) return;
-    const Scratch S = F->s;
-    const int n = F->d.count, ipt = F->ipt, nbr_cap = F->nbr_cap;
-    const size_t cap = (size_t)F->cap;
-    const auto nbr = gmem(S.nbr);
-    auto nbw = (__attribute__((address_space(1))) int *)S.nbr;
-    int *cursor = S.plan_src;  // zeroed by the reorder
-    const int base = cx.lb * kBlock * ipt;
-    for (int k = 0; k < ipt; ++k) {
-        const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= n) break;
-        const int u = min(S.nn[i], nbr_cap);
-        for (int q0 = 0; q0 < u; q0 += 4) {
-            // four independent entries, then their atomics, in flight together
-            int js[4], slot[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int q = q0 + t < u ? q0 + t : u - 1;
-                js[t] = nbr[(size_t)(nbr_cap - 1 - q) * cap + i];
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) slot[t] = q0 + t < u ? atomicAdd(cursor + js[t], 1) : nbr_cap;
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                if (slot[t] < nbr_cap) nbw[(size_t)slot[t] * cap + js[t]] = i;
-        }
-    }
-}
-
-// Sort network of 16 values (ascending), compare-exchange on fixed
-// registers: bitonic merges of 2, 4, 8, 16.
-__device__ __forceinline__ void sort16(int v[16]) {
-#pragma unroll
-    for (int k = 2; k <= 16; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-            for (int a = 0; a < 16; ++a) {
-                const int b = a ^ j;
-                if (b > a) {
-                    const bool up = (a & k) == 0;
-                    const int x = v[a], y = v[b];
-                    v[a] = up ? min(x, y) : max(x, y);
-                    v[b] = up ? max(x, y) : min(x, y);
-                }
-            }
-        }
-    }
-}
-
-// One bubble's full list: its lower entries (all below i) sorted at the
-// bottom, its upper list (all above i, ascending, stored top-down) moved down
-// right after them.  Length and degree are the total; a list over the
-// capacity is left to the rebuild.  Block partials: total, largest, overflow.
-__global__ __launch_bounds__(kBlock) void k_s_lists(Foam *foams, const int2 *bt) {
-    const Ctx cx = bubble_ctx(foams, bt);
-    Foam *F = cx.F;
-    if (!F->d.need_search) return;
-    const Scratch S = F->s;
-    const int n = F->d.count, ipt = F->ipt, nbr_cap = F->nbr_cap;
-    const size_t cap = (size_t)F->cap;
-    auto nbr = (__attribute__((address_space(1))) int *)S.nbr;
-    const int *low = S.del_list;
-    const int base = cx.lb * kBlock * ipt;
-    int sum_nn = 0, max_nn = 0, ovf = 0;
-    for (int k = 0; k < ipt; ++k) {
-        const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= n) break;
-        const int L = low[i], u = S.nn[i], tot = L + u;
-        sum_nn += tot;
-        max_nn = max(max_nn, tot);
-        if (tot > nbr_cap) {
-            ovf = 1;
-            S.nn[i] = nbr_cap;
-            S.deg[i] = nbr_cap;
-            continue;
-        }
-        auto at = [&](int q) -> __attribute__((address_space(1))) int & {
-            return nbr[(size_t)q * cap + i];
-        };
-        if (L <= 16) {
-            int v[16];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = q < L ? at(q) : 0x7fffffff;
-            if (L > 1) sort16(v);
-#pragma unroll
-            for (int q = 0; q < 16; ++q)
-                if (q < L) at(q) = v[q];
-        } else {
-            // long lower lists (rare): insertion sort in place
-            for (int q = 1; q < L; ++q) {
-                const int x = at(q);
-                int t = q - 1;
-                while (t >= 0 && at(t) > x) {
-                    at(t + 1) = at(t);
-                    --t;
-                }
-                at(t + 1) = x;
-            }
-        }
-        // upper entry q sits in column nbr_cap - 1 - q and goes to column L + q
-        if (u <= 16) {
-            int w[16];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) w[q] = q < u ? at(nbr_cap - 1 - q) : 0;
-#pragma unroll
-            for (int q = 0; q < 16; ++q)
-                if (q < u) at(L + q) = w[q];
-        } else {
-            // reverse in place at the top, then move down (the target starts
-            // at or below the source: a forward copy reads before it writes)
-            for (int q = 0; q < u / 2; ++q) {
-                const int a = nbr_cap - u + q, b = nbr_cap - 1 - q;
-                const int x = at(a);
-                at(a) = at(b);
-                at(b) = x;
-            }
-            for (int q = 0; q < u; ++q) at(L + q) = at(nbr_cap - u + q);
-        }
-        S.nn[i] = tot;
-        S.deg[i] = tot;
     }
     sum_nn = block_sum_i(sum_nn);
     max_nn = block_max_i(max_nn);
@@ -900,29 +743,13 @@ __global__ __launch_bounds__(kBlock) void k_s_lists(Foam *foams, const int2 *bt)
     }
 }
 
-// The lower-entry counts and cursors start at zero (the reorder clears them
-// in a full search; a list rebuild has no reorder).
-__global__ __launch_bounds__(kBlock) void k_s_zero(Foam *foams, const int2 *bt) {
-    const Ctx cx = bubble_ctx(foams, bt);
-    Foam *F = cx.F;
-    if (!F->d.need_search) return;
-    const int n = F->d.count, ipt = F->ipt;
-    const int base = cx.lb * kBlock * ipt;
-    for (int k = 0; k < ipt; ++k) {
-        const int i = base + k * kBlock + (int)threadIdx.x;
-        if (i >= n) break;
-        F->s.del_list[i] = 0;
-        F->s.plan_src[i] = 0;
-    }
-}
-
 // One block per foam: make the reordered buffers current (swap = 1), pair
 // count, list-capacity check, counters.
 __global__ __launch_bounds__(kBlock) void k_s_fin(Foam *foams, int swap) {
     Foam *F = foams + blockIdx.x;
     if (!F->d.need_search) return;
     const Scratch S = F->s;
-    const int nb = F->nblk;  // k_s_lists partials, per bubble-space block
+    const int nb = F->sblk;
     int s = 0, mx = 0, of = 0;
     for (int b = threadIdx.x; b < nb; b += kBlock) {
         s += S.cpart_i[b * kPartI + 1];
@@ -951,7 +778,7 @@ __global__ void k_set_flags(Foam *foams, int nf, int ns, int nr, int clear_halt)
     if (f >= nf) return;
     if (ns >= 0) foams[f].d.need_search = ns;
     if (nr >= 0) foams[f].d.need_remove = nr;
-    if (clear_halt && !foams[f].d.failed) {
+    if (clear_halt) {
         foams[f].d.halt = 0;
         foams[f].d.overflow = 0;
     }
@@ -964,7 +791,6 @@ __global__ void k_set_limits(Foam *foams, int nf, long long add) {
     if (f >= nf) return;
     Dyn &d = foams[f].d;
     d.step_limit = add > 0 ? d.steps + add : 0;
-    if (d.failed) return;  // stays halted
     d.halt = 0;
     d.overflow = 0;
 }
@@ -982,38 +808,22 @@ void launch_search(const Grid &g) {
     hipLaunchKernelGGL(k_s_reorder, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_s_search, dim3(g.nb_scell), dim3(kBlock), 0, g.stream, g.foams,
                        g.bt_scell);
-    hipLaunchKernelGGL(k_s_lower, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
-    hipLaunchKernelGGL(k_s_lists, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_s_fin, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, 1);
 }
 
 void launch_search_lists(const Grid &g) {
-    hipLaunchKernelGGL(k_s_zero, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_s_search, dim3(g.nb_scell), dim3(kBlock), 0, g.stream, g.foams,
                        g.bt_scell);
-    hipLaunchKernelGGL(k_s_lower, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
-    hipLaunchKernelGGL(k_s_lists, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
     hipLaunchKernelGGL(k_s_fin, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, 0);
 }
 
 void launch_search_part(const Grid &g, int part) {
-    switch (part) {
-    case kPartReorder:
+    if (kPartReorder == part)
         hipLaunchKernelGGL(k_s_reorder, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
                            g.bt_bub);
-        break;
-    case kPartListKernel:
+    else  // the list build is the list kernel alone
         hipLaunchKernelGGL(k_s_search, dim3(g.nb_scell), dim3(kBlock), 0, g.stream, g.foams,
                            g.bt_scell);
-        break;
-    default:  // kPartLists: the list build of a rebuild, without the totals
-        hipLaunchKernelGGL(k_s_zero, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
-        hipLaunchKernelGGL(k_s_search, dim3(g.nb_scell), dim3(kBlock), 0, g.stream, g.foams,
-                           g.bt_scell);
-        hipLaunchKernelGGL(k_s_lower, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
-        hipLaunchKernelGGL(k_s_lists, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams, g.bt_bub);
-        break;
-    }
 }
 
 void launch_set_flags(const Grid &g, int need_search, int need_remove, int clear_halt) {
