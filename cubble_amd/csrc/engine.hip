@@ -466,13 +466,9 @@ Grid Engine::grid() const {
 void Engine::enqueue_attempt(uint32_t flags) {
     const Grid g = grid();
     launch_predict(g, flags);
-    if (CUBBLE_PREC_MIXED == precision) {
-        launch_pair_mixed(g, flags);
-        launch_sums_mixed(g, flags);
-    } else {
-        launch_pair(g, flags);
-        launch_sums(g);
-    }
+    if (CUBBLE_PREC_MIXED == precision) launch_pair_mixed(g, flags);
+    else launch_pair(g, flags);
+    launch_sums(g);
     launch_correct(g, flags, CUBBLE_PREC_MIXED == precision);
 }
 

@@ -111,8 +111,6 @@ struct Dyn {
     int32_t count_stop;  // halt once count <= count_stop after a removal (0 = off)
     int32_t failed;      // the host could not give the foam what it needs (list
                          // growth out of memory): halted for good
-    int32_t n_near;      // mixed pair kernel: workgroups with a pair at a branch
-                         // threshold in this attempt (k_sums_mixed redoes them)
     double dt;
     uint64_t time_int;
     double time_frac;

@@ -23,12 +23,11 @@ struct Grid {
 
 void launch_predict(const Grid &g, uint32_t flags);
 void launch_pair(const Grid &g, uint32_t flags);
-void launch_pair_mixed(const Grid &g, uint32_t flags);
+// The mixed pair kernel, then (fix) the FP64 re-decision of the waves that
+// met a pair at a branch threshold.
+void launch_pair_mixed(const Grid &g, uint32_t flags, bool fix = true);
 // S_A, S_Ar, S_OA, S_OAr from the predictor and pair partials (one block per foam)
 void launch_sums(const Grid &g);
-// The same after the mixed pair kernel, which first redoes the workgroups
-// that met a pair at a branch threshold with that branch decided in FP64.
-void launch_sums_mixed(const Grid &g, uint32_t flags);
 // corrector (mixed: the mixed-precision variant, for engines of that
 // precision), then (control) the dt controller / commit (one block per foam)
 void launch_correct(const Grid &g, uint32_t flags, bool mixed, bool control = true);

@@ -24,11 +24,12 @@
 // the imposed flow counts a larger neighbour only past the overlap test
 // (:281-307).  FP32 distances (relative error ~1e-7) would decide those two
 // branch points differently from FP64 for pairs that close to them, and one
-// such pair moves its bubbles' overlap and dR/dt by O(1).  A bubble with a
-// pair within a relative band of 2^-16 around either threshold is summed a
-// second time with those branches decided from the FP64 predictions, exactly
-// as the FP64 path decides them; the band is ~15x the FP32 error and holds
-// about 1e-5 of the pairs.
+// such pair moves its bubbles' overlap and dR/dt by O(1).  A wave whose
+// bubbles met a pair within a relative band of 2^-16 around either threshold
+// (the band is ~15x the FP32 error and holds about 1e-5 of the pairs) marks
+// itself, and a second kernel sums the marked waves' bubbles again with those
+// branches decided from the FP64 predictions, exactly as the FP64 path
+// decides them, in parallel over the marked waves of every foam.
 #include "dev_util.hpp"
 #include "launch.hpp"
 
@@ -171,20 +172,29 @@ __device__ __forceinline__ float mixed_bubble(const Consts &c, const Roles &R, c
 
 // The bubbles of bubble-space block lb (thread t takes base + k kBlock + t,
 // k < ipt): pair terms and the block's overlap sums, per thread in k order.
+// EXACT: only the threads with `redo` sum again; the others take their
+// bubbles' stored overlap and radius, so the partials are those of a pass
+// deciding the marked waves' branches in FP64 from the start.
 template <int D, bool GAS, bool FLOW, bool EXACT>
 __device__ __forceinline__ void mixed_block(const Consts &c, const Roles &R, const Scratch &S,
                                             size_t cap, int base, int ipt, int n, double h,
-                                            double &soa, double &soar, bool &near_any) {
+                                            bool redo, double &soa, double &soar,
+                                            bool &near_any) {
     float step[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) step[a] = fixed_step(c.interval[a], c.wall[a] != 0);
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        float ri = 1.f;
-        bool near = false;
-        const float ovl = mixed_bubble<D, GAS, FLOW, EXACT>(c, R, S, cap, h, step, i, ri, near);
-        near_any = near_any || near;
+        float ri = 1.f, ovl;
+        if (!EXACT || redo) {
+            bool near = false;
+            ovl = mixed_bubble<D, GAS, FLOW, EXACT>(c, R, S, cap, h, step, i, ri, near);
+            near_any = near_any || near;
+        } else {
+            ovl = field<float>(S.ov)[i];
+            ri = load_rec(gmem(S.q4) + i).r;
+        }
         soa += (double)ovl;
         soar += (double)ovl * (1.0 / (double)ri);
     }
@@ -192,7 +202,8 @@ __device__ __forceinline__ void mixed_block(const Consts &c, const Roles &R, con
 
 template <bool FLOW, bool EXACT>
 __device__ __forceinline__ void mixed_block_dispatch(const Foam *F, int lb, uint32_t flags,
-                                                     double &soa, double &soar, bool &near_any) {
+                                                     bool redo, double &soa, double &soar,
+                                                     bool &near_any) {
     const int n = F->d.count, ipt = F->ipt;
     const Consts c = F->c;
     const Roles R = F->r;
@@ -202,19 +213,25 @@ __device__ __forceinline__ void mixed_block_dispatch(const Foam *F, int lb, uint
     const double h = 0.5 * F->d.dt;
     const bool gas = (flags & kGas) != 0;
     if (3 == c.dim) {
-        if (gas) mixed_block<3, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
-        else mixed_block<3, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        if (gas)
+            mixed_block<3, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, redo, soa, soar,
+                                              near_any);
+        else
+            mixed_block<3, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, redo, soa, soar,
+                                               near_any);
     } else {
-        if (gas) mixed_block<2, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
-        else mixed_block<2, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        if (gas)
+            mixed_block<2, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, redo, soa, soar,
+                                              near_any);
+        else
+            mixed_block<2, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, redo, soa, soar,
+                                               near_any);
     }
 }
 
 // One kernel per FLOW value (the flow variant needs more registers).  Every
-// branch is decided in FP32; a workgroup that met a pair within the band of
-// a threshold marks itself (block partial slot 2) and counts in the foam's
-// n_near, and k_sums_mixed recomputes its bubbles with those branches
-// decided in FP64.
+// branch is decided in FP32; each wave that met a pair within the band of a
+// threshold sets its bit in the block's mark (block partial slot 2).
 template <bool FLOW>
 __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *bt,
                                                        uint32_t flags) {
@@ -223,84 +240,58 @@ __global__ __launch_bounds__(kBlock) void k_pair_mixed(Foam *foams, const int2 *
     if (F->d.halt) return;
     double soa = 0.0, soar = 0.0;
     bool near = false;
-    mixed_block_dispatch<FLOW, false>(F, cx.lb, flags, soa, soar, near);
-    const int marked = __syncthreads_or(near ? 1 : 0);
+    mixed_block_dispatch<FLOW, false>(F, cx.lb, flags, true, soa, soar, near);
+    const bool wave_near = __builtin_amdgcn_ballot_w64(near) != 0ull;
+    const int wave_bit = 0 == (threadIdx.x & 63) && wave_near ? 1 << (threadIdx.x >> 6) : 0;
+    const int mark = block_sum_i(wave_bit);
     soa = block_sum(soa);
     soar = block_sum(soar);
     if (threadIdx.x == 0) {
         F->s.part[cx.lb * kPartD + 2] = soa;
         F->s.part[cx.lb * kPartD + 3] = soar;
-        F->s.part_i[cx.lb * kPartI + 2] = marked;
-        if (marked) atomicAdd(&F->d.n_near, 1);
+        F->s.part_i[cx.lb * kPartI + 2] = mark;
     }
 }
 
-void launch_pair_mixed(const Grid &g, uint32_t flags) {
-    if (flags & kFlow)
+// The marked waves of every foam, in parallel: their bubbles summed again
+// with the threshold branches decided in FP64 (the same bubbles per thread,
+// in the same order, so every stored value is that of a pass deciding them
+// so from the start), and the block's overlap partials formed again.  An
+// unmarked workgroup returns at once.
+template <bool FLOW>
+__global__ __launch_bounds__(kBlock) void k_fix_mixed(Foam *foams, const int2 *bt,
+                                                      uint32_t flags) {
+    const Ctx cx = bubble_ctx(foams, bt);
+    Foam *F = cx.F;
+    if (F->d.halt) return;
+    const int mark = F->s.part_i[cx.lb * kPartI + 2];
+    if (!mark) return;
+    const bool redo = ((mark >> (threadIdx.x >> 6)) & 1) != 0;
+    double soa = 0.0, soar = 0.0;
+    bool near = false;
+    mixed_block_dispatch<FLOW, true>(F, cx.lb, flags, redo, soa, soar, near);
+    soa = block_sum(soa);
+    soar = block_sum(soar);
+    if (threadIdx.x == 0) {
+        F->s.part[cx.lb * kPartD + 2] = soa;
+        F->s.part[cx.lb * kPartD + 3] = soar;
+    }
+}
+
+void launch_pair_mixed(const Grid &g, uint32_t flags, bool fix) {
+    if (flags & kFlow) {
         hipLaunchKernelGGL(k_pair_mixed<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
                            g.bt_bub, flags);
-    else
+        if (fix)
+            hipLaunchKernelGGL(k_fix_mixed<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream,
+                               g.foams, g.bt_bub, flags);
+    } else {
         hipLaunchKernelGGL(k_pair_mixed<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream,
                            g.foams, g.bt_bub, flags);
-}
-
-__device__ __forceinline__ double sum_parts(const double *part, int nb, int q) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) a += part[b * kPartD + q];
-    return block_sum(a);
-}
-
-// One block per foam, after k_pair_mixed: the workgroups it marked are
-// recomputed here with their threshold branches decided in FP64 -- the same
-// bubbles per thread, in the same order, so every stored value and the
-// block's overlap sums are those of a pass deciding them so from the start
-// -- then S_A, S_Ar, S_OA, S_OAr from the partials in a fixed order, as
-// k_sums does.
-__global__ __launch_bounds__(kBlock) void k_sums_mixed(Foam *foams, uint32_t flags) {
-    Foam *F = foams + blockIdx.x;
-    if (F->d.halt) return;
-    const int nb = F->nblk;
-    if (F->d.n_near > 0) {
-        __shared__ int list[kBlock];
-        for (int b0 = 0; b0 < nb; b0 += kBlock) {
-            const int b = b0 + (int)threadIdx.x;
-            const int mark = b < nb && F->s.part_i[b * kPartI + 2] ? 1 : 0;
-            int tot = 0;
-            const int pos = block_excl_scan(mark, tot);
-            if (mark) list[pos] = b;
-            __syncthreads();
-            for (int q = 0; q < tot; ++q) {
-                const int lb = list[q];
-                double soa = 0.0, soar = 0.0;
-                bool near = false;
-                if (flags & kFlow) mixed_block_dispatch<true, true>(F, lb, flags, soa, soar, near);
-                else mixed_block_dispatch<false, true>(F, lb, flags, soa, soar, near);
-                soa = block_sum(soa);
-                soar = block_sum(soar);
-                if (threadIdx.x == 0) {
-                    F->s.part[lb * kPartD + 2] = soa;
-                    F->s.part[lb * kPartD + 3] = soar;
-                }
-            }
-            __syncthreads();
-        }
+        if (fix)
+            hipLaunchKernelGGL(k_fix_mixed<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream,
+                               g.foams, g.bt_bub, flags);
     }
-    const double *part = F->s.part;
-    const double sa = sum_parts(part, nb, 0);
-    const double sar = sum_parts(part, nb, 1);
-    const double s_oa = sum_parts(part, nb, 2);
-    const double s_oar = sum_parts(part, nb, 3);
-    if (threadIdx.x == 0) {
-        F->d.s_a = sa;
-        F->d.s_ar = sar;
-        F->d.s_oa = s_oa;
-        F->d.s_oar = s_oar;
-        F->d.n_near = 0;
-    }
-}
-
-void launch_sums_mixed(const Grid &g, uint32_t flags) {
-    hipLaunchKernelGGL(k_sums_mixed, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
 }
 
 }  // namespace cubble_amd

@@ -130,14 +130,15 @@ int time_kernels(Engine &e, const cubble_integrate_params *ip, int reps, cubble_
     R_TRY(e, hipEventRecord(x[0], e.stream));
     for (int r = 0; r < reps; ++r) launch_predict(g, flags);
     R_TRY(e, hipEventRecord(x[1], e.stream));
+    // (mixed: the pair kernel alone, without the threshold re-decision pass)
     for (int r = 0; r < reps; ++r) {
-        if (CUBBLE_PREC_MIXED == e.precision) launch_pair_mixed(g, flags);
+        if (CUBBLE_PREC_MIXED == e.precision) launch_pair_mixed(g, flags, false);
         else launch_pair(g, flags);
     }
     R_TRY(e, hipEventRecord(x[2], e.stream));
     // the corrector alone (its inputs: the sums, reduced once before)
-    if (CUBBLE_PREC_MIXED == e.precision) launch_sums_mixed(g, flags);
-    else launch_sums(g);
+    if (CUBBLE_PREC_MIXED == e.precision) launch_pair_mixed(g, flags);
+    launch_sums(g);
     R_TRY(e, hipEventRecord(x[3], e.stream));
     for (int r = 0; r < reps; ++r) launch_correct(g, flags, CUBBLE_PREC_MIXED == e.precision, false);
     R_TRY(e, hipEventRecord(x[4], e.stream));
