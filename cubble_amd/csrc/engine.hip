@@ -460,6 +460,9 @@ Grid Engine::grid() const {
     g.nb_scell = nb_scell_;
     g.nfoams = (int)foams.size();
     g.stream = stream;
+    g.walls = false;
+    for (const FoamRec &r : foams)
+        for (int a = 0; a < r.h.c.dim; ++a) g.walls = g.walls || r.h.c.wall[a] != 0;
     return g;
 }
 
@@ -480,7 +483,11 @@ void Engine::enqueue_group(uint32_t flags) {
 }
 
 int Engine::capture(uint32_t flags, int groups) {
-    if (graph_ && graph_flags_ == flags && graph_groups_ == groups) return CUBBLE_OK;
+    // the graph holds the kernel variants chosen for the foams it was
+    // captured with (walls or not)
+    const bool walls = grid().walls;
+    if (graph_ && graph_flags_ == flags && graph_groups_ == groups && graph_walls_ == walls)
+        return CUBBLE_OK;
     if (graph_) {
         (void)hipGraphExecDestroy(graph_);
         graph_ = nullptr;
@@ -497,6 +504,7 @@ int Engine::capture(uint32_t flags, int groups) {
     }
     graph_flags_ = flags;
     graph_groups_ = groups;
+    graph_walls_ = walls;
     return CUBBLE_OK;
 }
 

@@ -170,6 +170,7 @@ class Engine {
     hipGraphExec_t graph_ = nullptr;
     uint32_t graph_flags_ = 0;
     int graph_groups_ = 0;
+    bool graph_walls_ = false;
     int committed_nf_ = 0;  // foams in the device array at the last commit
 
   public:

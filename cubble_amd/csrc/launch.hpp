@@ -19,6 +19,7 @@ struct Grid {
     int nb_scell;
     int nfoams;
     hipStream_t stream;
+    bool walls;  // some foam of the launch has a wall
 };
 
 void launch_predict(const Grid &g, uint32_t flags);

@@ -281,14 +281,15 @@ struct CorrAcc {
 };
 
 // MIXED: no FP64 predicted record is stored (recompute it) and velocities
-// are FP32.
-template <int D, bool GAS, bool PATH, bool MIXED>
+// are FP32.  WALLS false: no foam of the launch has a wall (the wall code,
+// and its registers, compiled out).
+template <int D, bool GAS, bool PATH, bool MIXED, bool WALLS>
 __device__ __forceinline__ void correct_range(const Consts &c, const Roles &R, const Scratch &S,
                                               const Dyn &dyn, int base, int ipt, int n,
                                               CorrAcc &a) {
     typedef typename std::conditional<MIXED, float, double>::type V;
     const double h = 0.5 * dyn.dt;
-    const bool walls = c.wall[0] || c.wall[1] || (3 == D && c.wall[2]);
+    const bool walls = WALLS && (c.wall[0] || c.wall[1] || (3 == D && c.wall[2]));
     const double inv_rho = (dyn.s_ar - dyn.s_oar) / (dyn.s_a - dyn.s_oa);
     const double flux = c.kappa * c.avg_area_in * n / dyn.s_a;  // same for every bubble
     for (int k = 0; k < ipt; ++k) {
@@ -427,23 +428,24 @@ __device__ void control(Foam *F, uint32_t flags, const CorrAcc &a) {
     if (stop_reached(d)) d.halt = 1;
 }
 
-template <int D, bool M>
+template <int D, bool M, bool W>
 __device__ __forceinline__ void correct_dispatch(const Consts &c, const Roles &R, const Scratch &S,
                                                  const Dyn &dyn, int base, int ipt, int n,
                                                  uint32_t flags, CorrAcc &a) {
     const bool gas = (flags & kGas) != 0, path = (flags & kPath) != 0;
     if (gas) {
-        if (path) correct_range<D, true, true, M>(c, R, S, dyn, base, ipt, n, a);
-        else correct_range<D, true, false, M>(c, R, S, dyn, base, ipt, n, a);
+        if (path) correct_range<D, true, true, M, W>(c, R, S, dyn, base, ipt, n, a);
+        else correct_range<D, true, false, M, W>(c, R, S, dyn, base, ipt, n, a);
     } else {
-        if (path) correct_range<D, false, true, M>(c, R, S, dyn, base, ipt, n, a);
-        else correct_range<D, false, false, M>(c, R, S, dyn, base, ipt, n, a);
+        if (path) correct_range<D, false, true, M, W>(c, R, S, dyn, base, ipt, n, a);
+        else correct_range<D, false, false, M, W>(c, R, S, dyn, base, ipt, n, a);
     }
 }
 
 // One kernel per precision (MIXED: FP32 velocity buffers, recomputed
-// prediction), so that neither pays the other's registers.
-template <bool MIXED>
+// prediction) and per wall presence (WALLS: some foam of the launch has a
+// wall), so that none pays the registers of another.
+template <bool MIXED, bool WALLS>
 __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt, uint32_t flags) {
     const Ctx cx = bubble_ctx(foams, bt);
     Foam *F = cx.F;
@@ -455,8 +457,8 @@ __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt,
     const Dyn dyn = F->d;
     const int base = cx.lb * kBlock * ipt;
     CorrAcc a{0.0, 0.0, 0.0, 0.0, 0};
-    if (3 == c.dim) correct_dispatch<3, MIXED>(c, R, S, dyn, base, ipt, n, flags, a);
-    else correct_dispatch<2, MIXED>(c, R, S, dyn, base, ipt, n, flags, a);
+    if (3 == c.dim) correct_dispatch<3, MIXED, WALLS>(c, R, S, dyn, base, ipt, n, flags, a);
+    else correct_dispatch<2, MIXED, WALLS>(c, R, S, dyn, base, ipt, n, flags, a);
     const double s_v = block_sum(a.s_v);
     const double me = block_max(a.max_err);
     const double mr = block_max(a.max_r);
@@ -574,12 +576,22 @@ void launch_bootstrap(const Grid &g, bool mixed) {
 }
 
 void launch_correct(const Grid &g, uint32_t flags, bool mixed, bool control) {
-    if (mixed)
-        hipLaunchKernelGGL(k_correct<true>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
-                           g.bt_bub, flags);
-    else
-        hipLaunchKernelGGL(k_correct<false>, dim3(g.nb_bub), dim3(kBlock), 0, g.stream, g.foams,
-                           g.bt_bub, flags);
+    const dim3 grid(g.nb_bub), block(kBlock);
+    if (mixed) {
+        if (g.walls)
+            hipLaunchKernelGGL((k_correct<true, true>), grid, block, 0, g.stream, g.foams,
+                               g.bt_bub, flags);
+        else
+            hipLaunchKernelGGL((k_correct<true, false>), grid, block, 0, g.stream, g.foams,
+                               g.bt_bub, flags);
+    } else {
+        if (g.walls)
+            hipLaunchKernelGGL((k_correct<false, true>), grid, block, 0, g.stream, g.foams,
+                               g.bt_bub, flags);
+        else
+            hipLaunchKernelGGL((k_correct<false, false>), grid, block, 0, g.stream, g.foams,
+                               g.bt_bub, flags);
+    }
     if (control)
         hipLaunchKernelGGL(k_control, dim3(g.nfoams), dim3(kBlock), 0, g.stream, g.foams, flags);
 }
