@@ -132,18 +132,9 @@ __device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &
             }
         }
     };
-    if (EXACT) {
-        // the rare pass: one neighbour at a time, so that its FP64 work does
-        // not add to the registers of the pipelined pass
-        const auto ids = gmem(nbr);
-        const auto recs = gmem(q4);
-        for (int q = 0; q < m_nn; ++q) {
-            const int j = ids[(size_t)q * cap + i];
-            body(j, load_rec(recs + j));
-        }
-    } else {
-        for_neighbours(nbr, q4, cap, i, m_nn, body);
-    }
+    // (the EXACT pass runs in its own kernel, so its FP64 work does not add
+    // to the registers of the main pass)
+    for_neighbours(nbr, q4, cap, i, m_nn, body);
     return near;
 }
 

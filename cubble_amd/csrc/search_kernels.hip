@@ -678,7 +678,11 @@ __device__ __forceinline__ int block_lower_bound(const int *__restrict__ a, int 
 // each for its own bubbles).  Tiles sized in bubbles keep the grid
 // proportional to the bubbles of a foam whatever its cell occupancy (2D
 // cells hold a few bubbles, crowded 3D ones a hundred and more).
-__global__ __launch_bounds__(kBlock) void k_s_search(Foam *foams, const int2 *bt) {
+// Eight waves per SIMD: left to itself the compiler keeps more scalar
+// registers (the candidate runs, the own records) than allow that, and the
+// kernel ran at seven (4-6% slower in the batch's searches).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_s_search(
+    Foam *foams, const int2 *bt) {
     const Ctx cx = cell_ctx(foams, bt);
     Foam *F = cx.F;
     if (!F->d.need_search) return;
