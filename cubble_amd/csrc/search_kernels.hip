@@ -573,6 +573,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             }
         }
         const unsigned long long own_all = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
+        const auto own = x4s + cb;  // the group's own records (scalar loads)
         for (int p0 = 0; p0 < total; p0 += 64) {
             // rows this chunk of candidates touches (scalar)
             int rows = 0;
@@ -597,10 +598,15 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             }
             const bool valid = p < total;
             const int j = valid ? p + o : -1;
-            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
             const int jl = valid ? j : i0;
-            const D4 pj = D4{x4[jl].x, x4[jl].y, 3 == D ? x4[jl].z : 0.0, x4[jl].r};
+            // a lane past the end holds a candidate at infinity: its
+            // distance test is false, so no validity mask enters the test
+            const D4 pj = D4{valid ? x4[jl].x : __builtin_inf(), x4[jl].y,
+                             3 == D ? x4[jl].z : 0.0, x4[jl].r};
             const bool anyflip = __builtin_amdgcn_ballot_w64(valid && f != 0) != 0ull;
+            // the self test only where the chunk holds some of the own bubbles
+            const bool anyself =
+                __builtin_amdgcn_ballot_w64(valid && j >= cb && j < cb + nb) != 0ull;
             // image shift of this lane's candidate per axis: 0, +L or -L
             double sh[3];
 #pragma unroll
@@ -613,9 +619,10 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
             // hit mask of own bubble i against the chunk (one ballot per
             // comparison: each is the comparison's own lane mask, with no
             // boolean materialised in a register)
-            auto test = [&](auto flip, int i) {
-                const double px = x4s[i].x, py = x4s[i].y, pr = x4s[i].r;
-                const double pz = 3 == D ? x4s[i].z : 0.0;
+            auto test = [&](auto flip, auto self, unsigned ia) {
+                // unsigned index: no sign extension in the scalar address
+                const double px = own[ia].x, py = own[ia].y, pr = own[ia].r;
+                const double pz = 3 == D ? own[ia].z : 0.0;
                 double d[3] = {px - pj.x, py - pj.y, 3 == D ? pz - pj.z : 0.0};
                 if (decltype(flip)::value) {
                     // d - (+-L) rounds exactly as -+(L - |d|); d - 0 is d
@@ -623,12 +630,15 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                     for (int k = 0; k < D; ++k) d[k] = d[k] - sh[k];
                 }
                 const double maxd = pr + pj.r + skin;
-                return vmask & __builtin_amdgcn_ballot_w64(j != i) &
-                       __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
+                const unsigned long long hit =
+                    __builtin_amdgcn_ballot_w64(sq_len(d) < maxd * maxd);
+                if (decltype(self)::value)
+                    return hit & __builtin_amdgcn_ballot_w64(j != cb + (int)ia);
+                return hit;
             };
             // two own bubbles per iteration: half the loop and load overhead on
             // the scalar unit, two independent FP64 chains in flight
-            auto own_loop = [&](auto flip) {
+            auto own_loop = [&](auto flip, auto self) {
                 unsigned long long todo = own_mask;
                 while (todo) {
                     const int ia = __builtin_ctzll(todo);
@@ -636,18 +646,23 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                     if (todo) {
                         const int ib = __builtin_ctzll(todo);
                         todo &= todo - 1ull;
-                        const unsigned long long ma = test(flip, cb + ia);
-                        const unsigned long long mb = test(flip, cb + ib);
+                        const unsigned long long ma = test(flip, self, (unsigned)ia);
+                        const unsigned long long mb = test(flip, self, (unsigned)ib);
                         if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
                         if (mb) append_hits(mb, ib, cb + ib, j, nbr, (unsigned)cap, nbr_cap, cnt);
                     } else {
-                        const unsigned long long ma = test(flip, cb + ia);
+                        const unsigned long long ma = test(flip, self, (unsigned)ia);
                         if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
                     }
                 }
             };
-            if (anyflip) own_loop(std::true_type{});
-            else own_loop(std::false_type{});
+            if (anyflip) {
+                if (anyself) own_loop(std::true_type{}, std::true_type{});
+                else own_loop(std::true_type{}, std::false_type{});
+            } else {
+                if (anyself) own_loop(std::false_type{}, std::true_type{});
+                else own_loop(std::false_type{}, std::false_type{});
+            }
         }
         finish_group(cb, nb, cnt, nbr_cap, nn, deg, sum_nn, max_nn, ovf);
     }
