@@ -343,7 +343,6 @@ __device__ __forceinline__ XRuns x_runs(const AxisCells &X) {
 __device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i, int j,
                                             int *__restrict__ nbr_, unsigned cap, int nbr_cap,
                                             int &cnt) {
-    const int lane = (int)(threadIdx.x & 63);
     const int base = __builtin_amdgcn_readlane(cnt, ii);
     const int upd = base + __popcll(m);  // scalar
     // the hit lanes become the exec mask directly (no per-lane bit test)
@@ -352,10 +351,12 @@ __device__ __forceinline__ void append_hits(unsigned long long m, int ii, int i,
             (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         const int pos = base + below;
         auto nbr = (__attribute__((address_space(1))) int *)(nbr_ + i);
-        if (upd <= nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;  // all fit (scalar test)
-        else if (pos < nbr_cap) nbr[(uint64_t)(unsigned)pos * cap] = j;
+        // slots past the capacity are counted, not written; when all fit (a
+        // scalar test) the bound is one no slot reaches
+        const int lim = upd <= nbr_cap ? 0x7fffffff : nbr_cap;
+        if (pos < lim) nbr[(uint64_t)(unsigned)pos * cap] = j;
     }
-    cnt = lane == ii ? upd : cnt;
+    cnt = (int)(threadIdx.x & 63) == ii ? upd : cnt;  // lane ii keeps the count
 }
 
 // The list length goes to nn and, as the degree the imposed flow divides by,
@@ -574,6 +575,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
         }
         const unsigned long long own_all = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
         const auto own = x4s + cb;  // the group's own records (scalar loads)
+        int *const nbr_own = nbr + cb;  // their list columns
         for (int p0 = 0; p0 < total; p0 += 64) {
             // rows this chunk of candidates touches (scalar)
             int rows = 0;
@@ -648,11 +650,11 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                         todo &= todo - 1ull;
                         const unsigned long long ma = test(flip, self, (unsigned)ia);
                         const unsigned long long mb = test(flip, self, (unsigned)ib);
-                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
-                        if (mb) append_hits(mb, ib, cb + ib, j, nbr, (unsigned)cap, nbr_cap, cnt);
+                        if (ma) append_hits(ma, ia, ia, j, nbr_own, (unsigned)cap, nbr_cap, cnt);
+                        if (mb) append_hits(mb, ib, ib, j, nbr_own, (unsigned)cap, nbr_cap, cnt);
                     } else {
                         const unsigned long long ma = test(flip, self, (unsigned)ia);
-                        if (ma) append_hits(ma, ia, cb + ia, j, nbr, (unsigned)cap, nbr_cap, cnt);
+                        if (ma) append_hits(ma, ia, ia, j, nbr_own, (unsigned)cap, nbr_cap, cnt);
                     }
                 }
             };
