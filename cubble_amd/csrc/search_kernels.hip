@@ -508,10 +508,36 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     const int fx0 = side(xr.a0, ccx), fx1 = side(xr.a1, ccx);
     // 2D foams have one z-row: 6 runs
     constexpr int kRuns = 3 == D ? kMaxRuns : kMaxRuns / 3;
+    // 2D: the runs' starts, offsets and image codes in scalar registers;
+    // 3D: one run per lane (lane s < kRuns holds run s, the lanes above it
+    // empty runs at the end), read by the chunks with readlane
     int off[kRuns], pre[kRuns], fl[kRuns];
+    int lane_pre = 0, lane_off = 0, lane_fl = 0;
     int total = 0;
+    if (3 == D) {
+        const int r = lane >> 1, part = lane & 1;
+        const int a = r / 3, b = r % 3;
+        const bool ok = lane < kRuns && a < Z.n && b < Y.n && (0 == part || xr.two);
+        const int zc = Z.at(a), yc = Y.at(b);
+        const int row = zc * cd[0] * cd[1] + yc * cd[0];
+        const auto vstart = gmem(start_);
+        const int js = ok ? vstart[row + (0 == part ? xr.a0 : xr.a1)] : 0;
+        const int je = ok ? vstart[row + (0 == part ? xr.b0 : xr.b1) + 1] : 0;
+        const int len = je - js;
+        // run starts: the exclusive prefix of the lengths over the lanes
+        int incl = len;
 #pragma unroll
-    for (int a = 0; a < (3 == D ? 3 : 1); ++a) {
+        for (int o = 1; o < 32; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        lane_pre = incl - len;
+        total = __builtin_amdgcn_readlane(incl, kRuns - 1);
+        lane_off = js - lane_pre;  // candidate p of the run is bubble p + off
+        lane_fl = (side(yc, ccy) << 2) | (side(zc, ccz) << 4) | (0 == part ? fx0 : fx1);
+    }
+#pragma unroll
+    for (int a = 0; a < (3 == D ? 0 : 1); ++a) {
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const bool row_ok = a < Z.n && b < Y.n;
@@ -576,28 +602,50 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
         const unsigned long long own_all = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
         const auto own = x4s + cb;  // the group's own records (scalar loads)
         int *const nbr_own = nbr + cb;  // their list columns
+        int s_cur = 0;  // 3D: the last non-empty run the previous chunk reached
         for (int p0 = 0; p0 < total; p0 += 64) {
-            // rows this chunk of candidates touches (scalar)
-            int rows = 0;
+            const int p = p0 + lane;
+            // the rows this chunk of candidates touches, and the run of
+            // candidate p: the last non-empty one starting at or before p
+            int rows = 0, o = 0, f = 0;
+            if (3 == D) {
+                // only the runs that start before the chunk's end, from the
+                // one holding p0 on (scalar loop, readlane of the tables)
+                int sr = s_cur;
+                int rs = __builtin_amdgcn_readlane(lane_pre, sr);
+                while (sr < kRuns && rs < p0 + 64) {
+                    const int re = __builtin_amdgcn_readlane(lane_pre, sr + 1);
+                    if (re > rs && re > p0) {
+                        const int os = __builtin_amdgcn_readlane(lane_off, sr);
+                        const int fs = __builtin_amdgcn_readlane(lane_fl, sr);
+                        const bool in = p >= rs;
+                        o = in ? os : o;
+                        f = in ? fs : f;
+                        rows |= 1 << (sr >> 1);
+                        s_cur = sr;
+                    }
+                    ++sr;
+                    rs = re;
+                }
+            } else {
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) {
-                const int rs = pre[2 * r];
-                const int re = 2 * r + 2 < kRuns ? pre[2 * r + 2] : total;
-                if (rs < re && rs < p0 + 64 && re > p0) rows |= 1 << r;
+                for (int r = 0; r < kRows; ++r) {
+                    const int rs = pre[2 * r];
+                    const int re = 2 * r + 2 < kRuns ? pre[2 * r + 2] : total;
+                    if (rs < re && rs < p0 + 64 && re > p0) rows |= 1 << r;
+                }
+                o = off[0];
+                f = fl[0];
+#pragma unroll
+                for (int s = 1; s < kRuns; ++s) {
+                    const bool in = p >= pre[s];
+                    o = in ? off[s] : o;
+                    f = in ? fl[s] : f;
+                }
             }
             const unsigned long long own_mask =
                 own_all & __builtin_amdgcn_ballot_w64((reach_rows & rows) != 0);
             if (!own_mask) continue;
-            const int p = p0 + lane;
-            // the run of candidate p is the last one starting at or before
-            // p (empty runs start where the next one does and are skipped)
-            int o = off[0], f = fl[0];
-#pragma unroll
-            for (int s = 1; s < kRuns; ++s) {
-                const bool in = p >= pre[s];
-                o = in ? off[s] : o;
-                f = in ? fl[s] : f;
-            }
             const bool valid = p < total;
             const int j = valid ? p + o : -1;
             const int jl = valid ? j : i0;
