@@ -468,7 +468,7 @@ constexpr int kOwnUnit = 32;  // own bubbles per search work unit
 // difference, without its test.  Same order, same lists, same rounding as
 // search_cell_general.
 //
-// Pruning by reach: an own bubble i cannot hit a candidate in the
+// Pruning by reach (3D): an own bubble i cannot hit a candidate in the
 // neighbouring row of cells below it in z (or y) when its distance to that
 // face of its cell is at least r_i + rmax + skin, rmax the largest radius of
 // the foam (all candidates lie beyond the face).  Each lane holds, for one
@@ -574,7 +574,9 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     const double ylo = c.lbb[1] + ccy * ey, yhi = ylo + ey;
     const double zlo = c.lbb[2] + ccz * ez, zhi = zlo + ez;
     const double slack = 1e-9 * fmax(L[0], fmax(L[1], L[2]));
-    const bool prune = rmax > 0.0;
+    // 2D cells hold about three bubbles, so a cell's candidates are one chunk
+    // touching all three rows, which no own bubble can skip: pruning in 3D only
+    const bool prune = 3 == D && rmax > 0.0;
     for (int cb = i0; cb < i1; cb += 64) {
         const int nb = uniform_i(min(64, i1 - cb));
         int cnt = 0;
@@ -628,12 +630,7 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                     rs = re;
                 }
             } else {
-#pragma unroll
-                for (int r = 0; r < kRows; ++r) {
-                    const int rs = pre[2 * r];
-                    const int re = 2 * r + 2 < kRuns ? pre[2 * r + 2] : total;
-                    if (rs < re && rs < p0 + 64 && re > p0) rows |= 1 << r;
-                }
+                rows = (1 << kRows) - 1;  // no pruning: every own bubble takes every chunk
                 o = off[0];
                 f = fl[0];
 #pragma unroll
