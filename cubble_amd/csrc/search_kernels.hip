@@ -334,6 +334,32 @@ __device__ __forceinline__ XRuns x_runs(const AxisCells &X) {
     return r;
 }
 
+// The x-cells [x0 - 1, x0 + k] around k consecutive own cells x0 .. x0 + k - 1
+// of one row, as at most two runs in ascending order, each with its image
+// code as `side` gives it for a neighbour cell (0; 1: the run lies past the
+// high end, its candidates take d - L; 2: past the low end, d + L).  On a
+// periodic axis k + 4 <= n, so the window wraps at one end at most and a
+// cell appears once.  A pair of cells more than one apart in the window is
+// at least a cell width apart under either image, so it never holds a pair
+// within reach.
+struct XWindow {
+    int a0, b0, f0, a1, b1, f1;
+    bool two;
+};
+
+__device__ __forceinline__ XWindow x_window(int x0, int k, int n, bool wall) {
+    XWindow r{x0 - 1, x0 + k, 0, 0, -1, 0, false};
+    if (wall) {
+        r.a0 = max(r.a0, 0);
+        r.b0 = min(r.b0, n - 1);
+    } else if (x0 - 1 < 0) {  // [0, x0 + k], then cell n - 1 past the low end
+        r = XWindow{0, x0 + k, 0, n - 1, n - 1, 2, true};
+    } else if (x0 + k >= n) {  // cells past the high end first, then [x0 - 1, n - 1]
+        r = XWindow{0, x0 + k - n, 1, x0 - 1, n - 1, 0, true};
+    }
+    return r;
+}
+
 // Wave-uniform append of the ballot `m` (non-zero) of candidates j to own
 // bubble i's list.  Lane `ii` of the group (ii is wave-uniform) keeps the
 // count: it is read with readlane and updated on the scalar unit.  A hit
@@ -452,6 +478,7 @@ __device__ __forceinline__ void search_cell_general(const Consts &c, const D4 *_
 
 constexpr int kMaxRuns = 18;  // 3 z-rows x 3 y-rows x 2 x-runs
 constexpr int kOwnUnit = 32;  // own bubbles per search work unit
+constexpr int kWin2 = 4;      // 2D: own cells of one row per work unit
 
 // Neighbour lists of one cell when every periodic axis has at least 5 cells
 // (the common case).  The candidate runs of the cell (at most two per row:
@@ -484,13 +511,14 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
                                             int *__restrict__ nbr, int *__restrict__ nn,
                                             int *__restrict__ deg,
                                             const int cd[3], size_t cap, int nbr_cap, int cell,
-                                            int i0, int i1, double rmax, int &sum_nn,
+                                            int kx, int i0, int i1, double rmax, int &sum_nn,
                                             int &max_nn, int &ovf) {
     const auto x4 = gmem(x4_);
     const auto x4s = cmem(x4_);
     const auto start = cmem(start_);
     const int lane = (int)(threadIdx.x & 63);
     cell = uniform_i(cell);
+    kx = uniform_i(kx);
     i0 = uniform_i(i0);
     i1 = uniform_i(i1);
     if (i0 >= i1) return;
@@ -536,6 +564,9 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
         lane_off = js - lane_pre;  // candidate p of the run is bubble p + off
         lane_fl = (side(yc, ccy) << 2) | (side(zc, ccz) << 4) | (0 == part ? fx0 : fx1);
     }
+    // 2D: the x-window of the unit's kx own cells of one row (kx = 1: the
+    // cell and its two neighbours, as in 3D)
+    const XWindow xw = x_window(ccx, kx, cd[0], c.wall[0] != 0);
 #pragma unroll
     for (int a = 0; a < (3 == D ? 0 : 1); ++a) {
 #pragma unroll
@@ -547,13 +578,13 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
 #pragma unroll
             for (int part = 0; part < 2; ++part) {
                 const int s = (a * 3 + b) * 2 + part;
-                const bool ok = row_ok && (0 == part || xr.two);
-                const int js = ok ? start[row + (0 == part ? xr.a0 : xr.a1)] : 0;
-                const int je = ok ? start[row + (0 == part ? xr.b0 : xr.b1) + 1] : 0;
+                const bool ok = row_ok && (0 == part || xw.two);
+                const int js = ok ? start[row + (0 == part ? xw.a0 : xw.a1)] : 0;
+                const int je = ok ? start[row + (0 == part ? xw.b0 : xw.b1) + 1] : 0;
                 // candidate p of the run is bubble p + off
                 off[s] = js - total;
                 pre[s] = total;
-                fl[s] = fyz | (0 == part ? fx0 : fx1);
+                fl[s] = fyz | (0 == part ? xw.f0 : xw.f1);
                 total += je - js;
             }
         }
@@ -774,30 +805,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     // bubbles and more once the radii spread) is shared by the waves of
     // several workgroups instead of serialising one wave; each own bubble's
     // list is still built by one wave, candidates in ascending order
+    //
+    // 2D cells hold about three bubbles, so a unit there takes up to kWin2
+    // consecutive cells of one row: its own bubbles share one candidate
+    // window (the rows above and below, the cells one past either end), so
+    // the unit's set-up and candidate chunk serve about kWin2 times as many
+    // own bubbles; a candidate more than one cell from an own bubble's cell
+    // is out of its reach and never listed.
     const int w = (int)(threadIdx.x >> 6);
     const auto cstart = cmem(S.cell_start);
+    int win2 = 1;
+    if (2 == c.dim && fast) win2 = c.wall[0] ? kWin2 : min(kWin2, cd[0] - 4);
     int uid = 0;
-    for (int cell = base; cell < end; ++cell) {
-        const int s0 = max(cstart[cell], b0), s1 = min(cstart[cell + 1], b1);
+    for (int cell = base; cell < end;) {
+        // own cells [cell, cell + kx): one row, within the tile's cells
+        const int row_end = (cell / cd[0] + 1) * cd[0];
+        const int kx = max(1, min(win2, min(end, row_end) - cell));
+        const int s0 = max(cstart[cell], b0), s1 = min(cstart[cell + kx], b1);
         for (int o0 = s0; o0 < s1; o0 += kOwnUnit, ++uid) {
             if ((uid & (kWaves - 1)) != w) continue;
             const int o1 = min(o0 + kOwnUnit, s1);
             if (3 == c.dim) {
                 if (fast)
                     search_cell<3>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
-                                   F->nbr_cap, cell, o0, o1, rmax, sum_nn, max_nn, ovf);
+                                   F->nbr_cap, cell, 1, o0, o1, rmax, sum_nn, max_nn, ovf);
                 else
                     search_cell_general<3>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
                                            F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
             } else {
                 if (fast)
                     search_cell<2>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
-                                   F->nbr_cap, cell, o0, o1, rmax, sum_nn, max_nn, ovf);
+                                   F->nbr_cap, cell, kx, o0, o1, rmax, sum_nn, max_nn, ovf);
                 else
                     search_cell_general<2>(c, S.x4, S.cell_start, S.nbr, S.nn, S.deg, cd, cap,
                                            F->nbr_cap, cell, o0, o1, sum_nn, max_nn, ovf);
             }
         }
+        cell += kx;
     }
     sum_nn = block_sum_i(sum_nn);
     max_nn = block_max_i(max_nn);
