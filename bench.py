@@ -165,8 +165,15 @@ def leg_batch(ca, args, ip):
         b.retune(shapes)
         tune = {"chosen": shapes, "from_cache": False, "candidates_timed": 0, "fixed": True}
     else:
-        tune = b.autotune(ip, trial_steps=args.tune_steps, repeats=2,
+        # each trial runs as many steps as the timed run (at least 16, at most
+        # --tune-steps): a short run starts while searches are frequent and
+        # replays end more often, and favours other shapes than a long one
+        trial = max(16, min(args.tune_steps, args.steps))
+        tune = b.autotune(ip, trial_steps=trial, repeats=2 if trial >= 64 else 3,
                           cache_path=args.tuning_cache)
+        tune["trial_steps"] = trial
+    # the device loop of the chosen shapes is captured before the clock starts
+    b.run_steps(0, ip)
     log("batch: launch shapes in %.1f s: %s" % (time.perf_counter() - t, json.dumps(tune)))
     s0 = [b.scalars(k) for k in range(len(cfgs))]
     t0 = time.perf_counter()

@@ -682,8 +682,12 @@ int Engine::integrate(uint32_t flags, std::vector<Dyn> &d) {
 
 int Engine::run_steps(uint32_t flags, int64_t steps, std::vector<Dyn> &d,
                       const std::vector<int64_t> *per_foam) {
-    if (steps <= 0) return read_dyn_all(d);
     const int groups = std::max(1, tuning_.groups);
+    if (steps <= 0) {
+        // no steps: only prepare the device loop (capture its graph)
+        const int rc = foams.empty() ? CUBBLE_OK : capture(flags, groups);
+        return rc ? rc : read_dyn_all(d);
+    }
     int rc = capture(flags, groups);
     if (rc) return rc;
     if ((rc = set_limits(steps, per_foam))) return rc;

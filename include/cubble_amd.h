@@ -219,7 +219,9 @@ int cubble_integrate(cubble_sim *sim, const cubble_integrate_params *ip, cubble_
 
 /* integrate() repeated `num_steps` times, driven entirely on the device (no
  * host synchronisation per step).  Replaces the main loop's integrate calls
- * (src/simulator.cu:1008-1011).  Returns when the steps are done. */
+ * (src/simulator.cu:1008-1011).  Returns when the steps are done.
+ * num_steps = 0 only prepares the device loop for these parameters (captures
+ * its graph), so that a following run starts at once. */
 int cubble_run_steps(cubble_sim *sim, const cubble_integrate_params *ip, int64_t num_steps,
                      cubble_step_result *res);
 
@@ -362,7 +364,8 @@ int cubble_batch_size(const cubble_batch *b);
 /* One accepted integrate() step for every foam. per_foam: n results or NULL. */
 int cubble_batch_integrate(cubble_batch *b, const cubble_integrate_params *ip,
                            cubble_step_result *per_foam);
-/* `num_steps` accepted steps for every foam, device-driven. */
+/* `num_steps` accepted steps for every foam, device-driven (0: prepare only,
+ * as cubble_run_steps). */
 int cubble_batch_run_steps(cubble_batch *b, const cubble_integrate_params *ip, int64_t num_steps,
                            cubble_step_result *per_foam);
 int cubble_batch_get_scalars(cubble_batch *b, int32_t k, cubble_scalars *out);

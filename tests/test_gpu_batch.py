@@ -141,3 +141,22 @@ def test_retune_live_simulation_equals_created_with_shapes():
         a.retune({"search_tile": 8})
     a.close()
     b.close()
+
+
+def test_zero_step_run_prepares_without_stepping():
+    """run_steps(0) captures the device loop and changes no state; a run after
+    it equals a run without it, bit for bit."""
+    cfgs = mixed_configs()[:3]
+    a = ca.Batch(cfgs)
+    b = ca.Batch(cfgs)
+    before = [a.get_state(k) for k in range(len(cfgs))]
+    res = a.run_steps(0, GAS)
+    assert all(r["steps"] == 0 for r in res)
+    for k in range(len(cfgs)):
+        assert_same(a.get_state(k), before[k])
+    a.run_steps(15, GAS)
+    b.run_steps(15, GAS)
+    for k in range(len(cfgs)):
+        assert_same(a.get_state(k), b.get_state(k))
+    a.close()
+    b.close()
