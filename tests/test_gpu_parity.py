@@ -391,3 +391,55 @@ def test_rescale_box_matches_oracle(ora, dim, n, seed, rel):
     ratio = np.array(gsc["interval"][:dim]) / np.array(rel[:dim])
     assert np.allclose(ratio, ratio[0], rtol=1e-14)
     sim.close()
+
+
+# Boxes of a few cells per axis.  2D: periodic rows of 5 to 9 cells, where the
+# search's row windows narrow and wrap at either row end), a wall in x or y,
+# and boxes elongated along x (long rows): the lists equal the oracle's, and
+# the brute-force pair set, after the first search and after a search with
+# jittered positions pushed across the faces.
+SMALL_2D = [(80, 91, (0, 0, 0), (1.0, 1.0, 1.0)), (110, 92, (0, 0, 0), (1.0, 1.0, 1.0)),
+            (150, 93, (0, 0, 0), (1.0, 1.0, 1.0)), (200, 94, (0, 0, 0), (1.0, 1.0, 1.0)),
+            (300, 95, (1, 0, 0), (1.0, 1.0, 1.0)), (400, 96, (0, 1, 0), (2.0, 1.0, 1.0)),
+            (600, 97, (0, 0, 0), (3.0, 1.0, 1.0)), (2000, 98, (1, 0, 0), (4.0, 1.0, 1.0))]
+
+
+# 3D: 5 to 7 cells per axis (rows of two runs at either end), a wall, a
+# box elongated along x.
+SMALL_3D = [(1400, 99, (0, 0, 0), (1.0, 1.0, 1.0)), (2400, 100, (0, 0, 0), (1.0, 1.0, 1.0)),
+            (3000, 101, (0, 0, 1), (2.0, 1.0, 1.0))]
+
+
+@pytest.mark.parametrize("dim,n,seed,wall,rel",
+                         [(2,) + c for c in SMALL_2D] + [(3,) + c for c in SMALL_3D])
+def test_small_and_long_boxes_search_bit_exact(ora, dim, n, seed, wall, rel):
+    cfg = config(dim, n, seed=seed, wall=wall, rel_dim=rel)
+    sim = ca.Simulator(cfg)
+    f = oracle_for(ora, cfg)
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    print("box", dim, n, wall, rel, "cells", list(gsc["cell_dim"][:dim]), flush=True)
+    assert list(gsc["cell_dim"]) == list(osc["cell_dim"])
+    assert_same_structure(gst, ost)
+    assert np.array_equal(ca.half_pairs(gst["nbr_off"], gst["nbr"]), f.brute_pairs())
+    # jitter, push some bubbles across the periodic faces, search again
+    rng = np.random.default_rng(seed)
+    L = np.array(gsc["interval"][:dim])
+    gst["x"][:dim] += rng.normal(0.0, 0.3, size=(dim, n))
+    for a in range(dim):
+        if not wall[a]:
+            gst["x"][a][a * (n // 10): (a + 1) * (n // 10)] += L[a] * 0.999
+        else:
+            lo, hi = gsc["lbb"][a], gsc["tfr"][a]
+            gst["x"][a] = np.clip(gst["x"][a], lo + 0.5, hi - 0.5)
+    sim.set_state(gst, gsc)
+    f = oracle_from_sim(ora, sim)
+    sim.search_neighbors()
+    f.search()
+    gst, gsc = sim.get_state()
+    ost, osc = f.get_state()
+    assert_same_structure(gst, ost)
+    assert np.array_equal(gst["x"], ost["x"])
+    assert gsc["num_pairs"] == osc["num_pairs"]
+    assert np.array_equal(ca.half_pairs(gst["nbr_off"], gst["nbr"]), f.brute_pairs())
+    sim.close()
