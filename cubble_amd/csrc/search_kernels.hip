@@ -746,29 +746,9 @@ __device__ __forceinline__ void search_cell(const Consts &c, const D4 *__restric
     }
 }
 
-// First index c in [0, m] with a[c] >= key, or m + 1 when there is none; a is
-// non-decreasing.  The workgroup probes kBlock evenly spaced positions of
-// the remaining range per round, so the range shrinks kBlock-fold each round
-// (two rounds for ~65,000 cells).  Every thread returns the same value.
-__device__ __forceinline__ int block_lower_bound(const int *__restrict__ a, int m, int key) {
-    int lo = 0, hi = m + 1;  // the answer lies in [lo, hi]; position m + 1 counts as >= key
-    while (lo < hi) {
-        const int step = (hi - lo + kBlock) / kBlock;  // ceil((hi - lo + 1) / kBlock)
-        const int p = lo + (int)threadIdx.x * step;
-        const bool probe = p <= hi;
-        const bool ge = probe && (p > m || a[p] >= key);
-        const int first_ge = -block_max_i(ge ? -p : -(hi + 1));   // min over true probes
-        const int last_lt = block_max_i(probe && !ge ? p : lo - 1);
-        hi = first_ge <= hi ? first_ge : hi;
-        lo = last_lt + 1;
-    }
-    return lo;
-}
-
 // Search grid: tile t of a foam searches for the own bubbles [t B, t B + B),
-// B = stile bubbles; the cells holding them are found by a lower bound on
-// the cell starts (a cell cut by a tile boundary is walked by both tiles,
-// each for its own bubbles).  Tiles sized in bubbles keep the grid
+// B = stile bubbles, whatever cells hold them (a cell cut by a tile
+// boundary is walked by both tiles, each for its own bubbles).  Tiles sized in bubbles keep the grid
 // proportional to the bubbles of a foam whatever its cell occupancy (2D
 // cells hold a few bubbles, crowded 3D ones a hundred and more).
 // Eight waves per SIMD: left to itself the compiler keeps more scalar
@@ -781,17 +761,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     if (!F->d.need_search) return;
     const Consts c = F->c;
     const Scratch S = F->s;
-    const int ncell = F->d.num_cells;
     const int cd[3] = {F->d.cell_dim[0], F->d.cell_dim[1], F->d.cell_dim[2]};
     const double rmax = F->d.search_max_r;  // 0 when not computed (list rebuild)
     // tile t: own bubbles [t B, t B + B) (B = stile), whatever cells they lie
     // in; the cells holding them are [base, end)
     const int b0 = cx.lb * F->stile;
     const int b1 = min(b0 + F->stile, F->d.count);
+    // the cells holding them: position q of the cell order lies in the
+    // segment of the cell of members[q] (any member of that segment), so the
+    // first and last own bubbles name the range (scalar loads)
     int base = 0, end = 0;
     if (b0 < b1) {
-        base = block_lower_bound(S.cell_start, ncell, b0 + 1) - 1;  // the cell of bubble b0
-        end = min(block_lower_bound(S.cell_start, ncell, b1), ncell);
+        const auto mem = cmem(S.members);
+        const auto cof = cmem(S.cell_of);
+        base = cof[mem[b0]];
+        end = cof[mem[b1 - 1]] + 1;
     }
     const size_t cap = (size_t)F->cap;
     int sum_nn = 0, max_nn = 0, ovf = 0;
