@@ -126,6 +126,9 @@ __device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &
     const float close2 = close * close;
     // absolute part of the 2D tangency band
     const float tangent_s = kTangentS * s_max;
+    // the band around the inside threshold big = max(r1, r2)^2, at least
+    // band_abs max(r1, r2) + kBranchBand big
+    const float inside_k0 = 0.5f * band_abs, inside_k1 = 0.5f * band_abs + kBranchBand;
     const float inv_ri = __builtin_amdgcn_rcpf(ri_f);
     D4 pi{};
     if (EXACT) pi = predicted<D, GAS, float>(R, h, i);
@@ -155,9 +158,8 @@ __device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &
             const float t = fmaf(kTangentR, rad, tangent_s);
             special = special || fabsf(over) <= t * t;
         }
-        if (GAS)
-            special = special || (over >= 0.f && fabsf(m - big) <=
-                                                     band_abs * fmaxf(ri_f, rj_f) + kBranchBand * big);
+        // (the inside band uses max(r1, r2) <= (big + 1) / 2: one FMA)
+        if (GAS) special = special || fabsf(m - big) <= fmaf(big, inside_k1, inside_k0);
         bool overlap = over >= 0.f;
         bool inside = m < big;
         if (special) {
