@@ -104,38 +104,46 @@ struct MixedOut {
     float acc[3], fv[3], ovl, dr;
 };
 
-// One bubble's pair sums over its list, ascending.  EXACT = false evaluates
-// every pair in FP32 from code differences and reports whether a pair needs
-// FP64 values (close, near-tangent in 2D, or near a branch point); EXACT =
-// true (run only for such bubbles) takes those pairs' values and decisions
-// from the FP64 predictions, with everything else the same.
+// The terms of bubble i's pairs, one call per neighbour j.  EXACT = false
+// evaluates the pair in FP32 from code differences; EXACT = true takes the
+// values and decisions of a pair that needs them from the FP64 predictions
+// (pi, bubble i's), with everything else the same.  A call adds the pair's
+// terms to o and returns whether the pair needs FP64 values: nearly
+// coincident centres, near-tangent in 2D, or near a branch point.
 template <int D, bool GAS, bool FLOW, bool EXACT>
-__device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &R,
-                                                  const Q4 *__restrict__ q4,
-                                                  const int *__restrict__ nbr, size_t cap,
-                                                  double h, const float step[3], int i,
-                                                  const Q4 &qi, int m_nn, MixedOut &o) {
-    const float f0 = (float)c.f0;
-    const float pi_f = (float)kPi;
-    const float ri_f = qi.r;
-    // absolute part of the threshold band: the largest code unit
-    const float s_max = fmaxf(fmaxf(step[0], step[1]), D == 3 ? step[2] : 0.f);
-    const float band_abs = kBandSteps * s_max;
-    // distance below which a pair's d comes from the FP64 predictions
-    const float close = kCloseSteps * (D == 3 ? 1.7320508f : 1.4142135f) * s_max;
-    const float close2 = close * close;
-    // absolute part of the 2D tangency band
-    const float tangent_s = kTangentS * s_max;
-    // the band around the inside threshold big = max(r1, r2)^2, at least
-    // band_abs max(r1, r2) + kBranchBand big
-    const float inside_k0 = 0.5f * band_abs, inside_k1 = 0.5f * band_abs + kBranchBand;
-    const float inv_ri = __builtin_amdgcn_rcpf(ri_f);
-    D4 pi{};
-    if (EXACT) pi = predicted<D, GAS, float>(R, h, i);
-    bool near = false;
-    for (int a = 0; a < 3; ++a) o.acc[a] = o.fv[a] = 0.f;
-    o.ovl = o.dr = 0.f;
-    auto body = [&](int j, const Q4 &qj) {
+struct MixedPair {
+    const Consts &c;
+    const Roles &R;
+    const float *step;
+    double h;
+    int i;
+    Q4 qi;
+    D4 pi;
+    float f0, ri_f, inv_ri, band_abs, close2, tangent_s, inside_k0, inside_k1;
+
+    __device__ __forceinline__ MixedPair(const Consts &c_, const Roles &R_, double h_,
+                                         const float step_[3], int i_, const Q4 &qi_)
+        : c(c_), R(R_), step(step_), h(h_), i(i_), qi(qi_), pi{} {
+        f0 = (float)c.f0;
+        ri_f = qi.r;
+        inv_ri = __builtin_amdgcn_rcpf(ri_f);
+        // absolute part of the threshold band: the largest code unit
+        const float s_max = fmaxf(fmaxf(step[0], step[1]), D == 3 ? step[2] : 0.f);
+        band_abs = kBandSteps * s_max;
+        // distance below which a pair's d comes from the FP64 predictions
+        const float close = kCloseSteps * (D == 3 ? 1.7320508f : 1.4142135f) * s_max;
+        close2 = close * close;
+        // absolute part of the 2D tangency band
+        tangent_s = kTangentS * s_max;
+        // the band around the inside threshold big = max(r1, r2)^2, at least
+        // band_abs max(r1, r2) + kBranchBand big
+        inside_k0 = 0.5f * band_abs;
+        inside_k1 = 0.5f * band_abs + kBranchBand;
+        if (EXACT) pi = predicted<D, GAS, float>(R, h, i);
+    }
+
+    __device__ __forceinline__ bool operator()(int j, const Q4 &qj, MixedOut &o) const {
+        const float pi_f = (float)kPi;
         const float rj_f = qj.r;
         // code differences are exact integers; one rounding to FP32 each
         float dd[3] = {(float)fixed_diff(qi.x, qj.x) * step[0],
@@ -149,8 +157,7 @@ __device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &
         // in the radii, so their order does not matter
         const float sq_i = ri_f * ri_f, sq_j = rj_f * rj_f;
         const float big = fmaxf(sq_i, sq_j);
-        // pairs the FP32 values do not settle: nearly coincident centres,
-        // near-tangent pairs in 2D, and pairs near a branch point
+        // pairs the FP32 values do not settle
         bool special = m < close2;
         if (FLOW)
             special = special || (j > i && fabsf(over) <= band_abs * rad + kBranchBand * (rad * rad));
@@ -162,17 +169,14 @@ __device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &
         if (GAS) special = special || fabsf(m - big) <= fmaf(big, inside_k1, inside_k0);
         bool overlap = over >= 0.f;
         bool inside = m < big;
-        if (special) {
-            near = true;
-            if (EXACT) {
-                const ExactPair e = exact_pair<D, GAS>(c, R, h, pi, i, j);
+        if (EXACT && special) {
+            const ExactPair e = exact_pair<D, GAS>(c, R, h, pi, i, j);
 #pragma unroll
-                for (int a = 0; a < 3; ++a) dd[a] = e.dd[a];
-                m = e.m;
-                over = e.over;
-                overlap = e.overlap;
-                inside = e.inside;
-            }
+            for (int a = 0; a < 3; ++a) dd[a] = e.dd[a];
+            m = e.m;
+            over = e.over;
+            overlap = e.overlap;
+            inside = e.inside;
         }
         if (FLOW && (j < i || overlap)) {
 #pragma unroll
@@ -197,47 +201,37 @@ __device__ __forceinline__ bool pair_bubble_mixed(const Consts &c, const Roles &
                 o.dr += oa * (__builtin_amdgcn_rcpf(rj_f) - inv_ri);
             }
         }
-    };
-    // (the EXACT pass runs in its own kernel, so its FP64 work does not add
-    // to the registers of the main pass)
-    for_neighbours(nbr, q4, cap, i, m_nn, body);
-    return near;
+        return special;
+    }
+};
+
+__device__ __forceinline__ void zero_out(MixedOut &o) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) o.acc[a] = o.fv[a] = 0.f;
+    o.ovl = o.dr = 0.f;
 }
 
-// One bubble's pair terms, stored (velocities, dR/dt, overlap); returns its
-// overlap sum and radius for the block partials, and whether a pair came
-// a pair that needs FP64 values.  FLOW: also the imposed-flow term
-// (the neighbour-velocity sum in FP32, the flow-box test on the bubble's FP64
-// prediction, recomputed here as the corrector does); same neighbour
-// selection as the FP64 kernel.
-template <int D, bool GAS, bool FLOW, bool EXACT>
-__device__ __forceinline__ float mixed_bubble(const Consts &c, const Roles &R, const Scratch &S,
-                                              size_t cap, double h, const float step[3], int i,
-                                              float &ri, bool &near) {
-    const Q4 qi = load_rec(gmem(S.q4) + i);
-    MixedOut o;
-    near = pair_bubble_mixed<D, GAS, FLOW, EXACT>(c, R, S.q4, S.nbr, cap, h, step, i, qi,
-                                                  S.nn[i], o);
+// Bubble i's terms stored: velocities, dR/dt and overlap, the overlap's sign
+// bit set when `flag` (the overlap sum is never negative: the flag marks a
+// bubble that met a pair needing FP64 values, for the second kernel, which
+// stores it plain).  FLOW: the imposed-flow term is added first (the
+// neighbour-velocity sum in FP32, the flow-box test on the bubble's FP64
+// prediction, recomputed here as the corrector does).
+template <int D, bool GAS, bool FLOW>
+__device__ __forceinline__ void store_mixed(const Consts &c, const Roles &R, const Scratch &S,
+                                            double h, int i, MixedOut &o, bool flag) {
     if (FLOW) add_flow<D>(c, predicted<D, GAS, float>(R, h, i), S.deg[i], o.fv, o.acc);
 #pragma unroll
     for (int a = 0; a < D; ++a) stream_store(field<float>(R.vp[a]) + i, o.acc[a]);
     stream_store(field<float>(R.vp[3]) + i, o.dr);
-    // the overlap sum is never negative: in the main pass its sign bit flags a
-    // bubble that met a pair needing FP64 values, for the second kernel, whose
-    // pass stores it plain
     stream_store(field<float>(S.ov) + i,
-                 !EXACT && near ? __uint_as_float(__float_as_uint(o.ovl) | 0x80000000u)
-                                : o.ovl);
-    ri = qi.r;
-    return o.ovl;
+                 flag ? __uint_as_float(__float_as_uint(o.ovl) | 0x80000000u) : o.ovl);
 }
 
 // The bubbles of bubble-space block lb (thread t takes base + k kBlock + t,
-// k < ipt): pair terms and the block's overlap sums, per thread in k order.
-// EXACT: only the bubbles whose stored overlap carries the flag sum again;
-// the others take their stored overlap and radius, so the partials are those
-// of a pass taking the flagged pairs' values from FP64 from the start.
-template <int D, bool GAS, bool FLOW, bool EXACT>
+// k < ipt): each bubble's pair sums over its list, ascending, in FP32, its
+// terms stored, and the block's overlap sums, per thread in k order.
+template <int D, bool GAS, bool FLOW>
 __device__ __forceinline__ void mixed_block(const Consts &c, const Roles &R, const Scratch &S,
                                             size_t cap, int base, int ipt, int n, double h,
                                             double &soa, double &soar, bool &near_any) {
@@ -247,25 +241,104 @@ __device__ __forceinline__ void mixed_block(const Consts &c, const Roles &R, con
     for (int k = 0; k < ipt; ++k) {
         const int i = base + k * kBlock + (int)threadIdx.x;
         if (i >= n) break;
-        float ri = 1.f, ovl = 0.f;
-        bool redo = true;
-        if (EXACT) {
-            ovl = field<float>(S.ov)[i];
-            redo = (__float_as_uint(ovl) >> 31) != 0;
-        }
-        if (redo) {
-            bool near = false;
-            ovl = mixed_bubble<D, GAS, FLOW, EXACT>(c, R, S, cap, h, step, i, ri, near);
-            near_any = near_any || near;
-        } else {
-            ri = load_rec(gmem(S.q4) + i).r;
-        }
-        soa += (double)ovl;
-        soar += (double)ovl * (1.0 / (double)ri);
+        const Q4 qi = load_rec(gmem(S.q4) + i);
+        const MixedPair<D, GAS, FLOW, false> pair(c, R, h, step, i, qi);
+        MixedOut o;
+        zero_out(o);
+        bool near = false;
+        for_neighbours(S.nbr, S.q4, cap, i, S.nn[i],
+                       [&](int j, const Q4 &qj) { near = pair(j, qj, o) || near; });
+        store_mixed<D, GAS, FLOW>(c, R, S, h, i, o, near);
+        near_any = near_any || near;
+        soa += (double)o.ovl;
+        soar += (double)o.ovl * (1.0 / (double)qi.r);
     }
 }
 
-template <bool FLOW, bool EXACT>
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Flagged bubble i (wave-uniform) summed again by the whole wave, the values
+// of the pairs that need them taken from the FP64 predictions: lane l takes
+// list entries l, l + 64, ..., and the lanes' sums are added in a fixed
+// butterfly order (every lane ends with the same sums, independent of
+// timing).  Lane 0 stores the terms; returns the overlap sum.
+template <int D, bool GAS, bool FLOW>
+__device__ __forceinline__ float exact_bubble_wave(const Consts &c, const Roles &R,
+                                                   const Scratch &S, size_t cap, double h,
+                                                   const float step[3], int i) {
+    const Q4 qi = load_rec(gmem(S.q4) + i);
+    const MixedPair<D, GAS, FLOW, true> pair(c, R, h, step, i, qi);
+    MixedOut o;
+    zero_out(o);
+    const int m_nn = S.nn[i];
+    for (int q = (int)(threadIdx.x & 63); q < m_nn; q += 64) {
+        const int j = gmem(S.nbr)[(size_t)q * cap + i];
+        pair(j, load_rec(gmem(S.q4) + j), o);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        o.acc[a] = wave_sum_f(o.acc[a]);
+        if (FLOW) o.fv[a] = wave_sum_f(o.fv[a]);
+    }
+    if (GAS) {
+        o.ovl = wave_sum_f(o.ovl);
+        o.dr = wave_sum_f(o.dr);
+    }
+    if (0 == (threadIdx.x & 63)) store_mixed<D, GAS, FLOW>(c, R, S, h, i, o, false);
+    return o.ovl;
+}
+
+// The flagged bubbles of block lb, one wave per flagged bubble, and the
+// block's overlap sums formed again: the other bubbles' stored overlaps and
+// radii, the flagged ones' new overlaps, per thread in k order as the main
+// pass adds them.
+template <int D, bool GAS, bool FLOW>
+__device__ __forceinline__ void fix_block(const Consts &c, const Roles &R, const Scratch &S,
+                                          size_t cap, int base, int ipt, int n, double h,
+                                          double &soa, double &soar) {
+    __shared__ int s_cnt;
+    __shared__ int s_list[kBlock];
+    __shared__ float s_ovl[kBlock];
+    float step[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) step[a] = fixed_step(c.interval[a], c.wall[a] != 0);
+    const int wave = (int)(threadIdx.x >> 6);
+    for (int k = 0; k < ipt; ++k) {
+        const int i = base + k * kBlock + (int)threadIdx.x;
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        float ovl = 0.f, ri = 1.f;
+        int slot = -1;
+        if (i < n) {
+            ovl = field<float>(S.ov)[i];
+            ri = load_rec(gmem(S.q4) + i).r;
+            if (__float_as_uint(ovl) >> 31) {
+                slot = atomicAdd(&s_cnt, 1);
+                s_list[slot] = i;
+            }
+        }
+        __syncthreads();
+        const int cnt = s_cnt;
+        for (int f = wave; f < cnt; f += kBlock / 64) {
+            const float o = exact_bubble_wave<D, GAS, FLOW>(c, R, S, cap, h, step, s_list[f]);
+            if (0 == (threadIdx.x & 63)) s_ovl[f] = o;
+        }
+        __syncthreads();
+        if (slot >= 0) ovl = s_ovl[slot];
+        if (i < n) {
+            soa += (double)ovl;
+            soar += (double)ovl * (1.0 / (double)ri);
+        }
+    }
+}
+
+// Foam constants and the (dimension, gas) variant of block lb; FIX selects
+// the second kernel's block pass.
+template <bool FLOW, bool FIX>
 __device__ __forceinline__ void mixed_block_dispatch(const Foam *F, int lb, uint32_t flags,
                                                      double &soa, double &soar, bool &near_any) {
     const int n = F->d.count, ipt = F->ipt;
@@ -277,15 +350,21 @@ __device__ __forceinline__ void mixed_block_dispatch(const Foam *F, int lb, uint
     const double h = 0.5 * F->d.dt;
     const bool gas = (flags & kGas) != 0;
     if (3 == c.dim) {
-        if (gas)
-            mixed_block<3, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
-        else
-            mixed_block<3, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        if (gas) {
+            if (FIX) fix_block<3, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
+            else mixed_block<3, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        } else {
+            if (FIX) fix_block<3, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
+            else mixed_block<3, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        }
     } else {
-        if (gas)
-            mixed_block<2, true, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
-        else
-            mixed_block<2, false, FLOW, EXACT>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        if (gas) {
+            if (FIX) fix_block<2, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
+            else mixed_block<2, true, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        } else {
+            if (FIX) fix_block<2, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar);
+            else mixed_block<2, false, FLOW>(c, R, S, cap, base, ipt, n, h, soa, soar, near_any);
+        }
     }
 }
 
@@ -315,12 +394,10 @@ k_pair_mixed(Foam *foams, const int2 *bt, uint32_t flags) {
     }
 }
 
-// The flagged bubbles of every foam, in parallel: summed again with the
-// values and decisions of the pairs that need them taken from the FP64
-// predictions (the same bubbles per thread, in the same order, so every
-// stored value is that of a pass taking them so from the start), and their
-// blocks' overlap partials formed again.  An unmarked workgroup returns at
-// once; in a marked one only the lanes of flagged bubbles walk their lists.
+// The flagged bubbles of every foam, in parallel, one wave each: summed again
+// with the values and decisions of the pairs that need them taken from the
+// FP64 predictions, and their blocks' overlap partials formed again.  An
+// unmarked workgroup returns at once.
 template <bool FLOW>
 __global__ __launch_bounds__(kBlock) void k_fix_mixed(Foam *foams, const int2 *bt,
                                                       uint32_t flags) {
