@@ -247,6 +247,67 @@ def test_mixed_precision_attempt_error(ora, dim, n, seed, wall):
     sim.close()
 
 
+@pytest.mark.parametrize("dim,wall", [(2, (0, 0, 0)), (2, (1, 1, 0)), (3, (0, 0, 0)),
+                                      (3, (1, 0, 1))])
+def test_mixed_close_and_tangent_pairs(ora, dim, wall):
+    """Hand-placed pairs that FP32 values from the fixed-point codes do not
+    settle, 20 of each kind: centres 1e-6..1e-3 apart (quantisation turns the
+    pair direction by up to ~s / |d|, here up to ~1e-1), centres within
+    1e-9..1e-6 of tangency (in 2D the overlap term moves by up to ~sqrt of
+    the error of R^2 - |d|^2), and centres within 1e-9..1e-6 of the inside
+    point |d| = max(r1, r2) (the area jumps there).  Their bubbles are at
+    rest, so the predictions keep the placed distances.  The attempt meets
+    the 1e-5 bar, and the planted bubbles match the oracle to 1e-6."""
+    print(f"mixed planted pairs dim={dim} wall={wall}", flush=True)
+    n = 3000 if dim == 2 else 4000
+    sim = ca.Simulator(config(dim, n, seed=90 + dim, precision=ca.PREC_MIXED, wall=wall))
+    st, sc = sim.get_state()
+    rng = np.random.default_rng(90 + dim)
+    lo, hi = np.array(sc["lbb"][:dim]), np.array(sc["tfr"][:dim])
+    x = st["x"]
+    inner = np.flatnonzero(np.all((x[:dim].T > lo + 4.0) & (x[:dim].T < hi - 4.0), axis=1))
+    pick = rng.choice(inner, size=120, replace=False)
+    planted = []
+    for k in range(60):
+        a, b = pick[2 * k], pick[2 * k + 1]
+        ra, rb = x[3][a], x[3][b]
+        delta = 10.0 ** rng.uniform(-9, -6) * rng.choice([-1.0, 1.0])
+        if k % 3 == 0:
+            dist = 10.0 ** rng.uniform(-6, -3)
+        elif k % 3 == 1:
+            dist = ra + rb + delta
+        else:
+            dist = max(ra, rb) + delta
+        u = rng.normal(size=dim)
+        x[:dim, b] = x[:dim, a] + dist * u / np.linalg.norm(u)
+        for key in ("v", "vo"):
+            st[key][:, a] = 0.0
+            st[key][:, b] = 0.0
+        planted += [st["idx"][a], st["idx"][b]]
+    sim.set_state(st, sc)
+    sim.search_neighbors()
+    gst, _ = sim.get_state()
+    f = oracle_from_sim(ora, sim)
+    sim.step(GAS)
+    f.attempt(True, True)
+    g, o = sim.get_attempt(), f.attempt_out()
+    f0 = sc["f0"]
+    errs = attempt_errors(dim, g, o, f0)
+    mask = np.isin(gst["idx"], planted)
+    assert mask.sum() == 120
+    tight = {
+        "velocity": max(scaled_err(g["vp"][k][mask], o["vp"][k][mask], f0) for k in range(dim)),
+        "drdt": scaled_err(g["vp"][3][mask], o["vp"][3][mask], float(np.max(np.abs(o["vp"][3])))),
+        "overlap": scaled_err(g["ov"][mask], o["ov"][mask], float(np.max(np.abs(o["ov"])))),
+    }
+    print("mixed planted errors", errs, tight, flush=True)
+    for q, e in errs.items():
+        assert e <= MIXED_TOL, (q, e)
+    for q, e in tight.items():
+        assert e <= 1e-6, (q, e)
+    sim.close()
+
+
 def test_large_foam_multiblock_partials(ora):
     """More than 256 * 4096 bubbles: several bubbles per thread, the full
     reduction tree and the XCD block remap."""
