@@ -130,6 +130,41 @@ __device__ __forceinline__ int block_max_i(int v) {
     return t;
 }
 
+// Several block reductions in one LDS round: each wave reduces its values
+// with the butterflies above, lane 0 stores them, and after one barrier
+// thread 0 combines the waves in wave order, as block_sum, block_max and
+// block_sum_i do (the same bits).  Bit k of `dmax` makes d[k] a maximum,
+// else a sum; `iv` is summed.  The results are defined in thread 0 only.
+// The leading barrier guards the LDS against a previous call.
+template <int ND>
+__device__ __forceinline__ void block_reduce_t0(double (&d)[ND], unsigned dmax, int &iv) {
+    __shared__ double sd[kWaves][ND];
+    __shared__ int si[kWaves];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) d[k] = (dmax >> k) & 1u ? wave_max(d[k]) : wave_sum(d[k]);
+    iv = wave_sum_i(iv);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < ND; ++k) sd[threadIdx.x >> 6][k] = d[k];
+        si[threadIdx.x >> 6] = iv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            double t = (dmax >> k) & 1u ? sd[0][k] : 0.0 + sd[0][k];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) t = (dmax >> k) & 1u ? fmax(t, sd[w][k]) : t + sd[w][k];
+            d[k] = t;
+        }
+        int t = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += si[w];
+        iv = t;
+    }
+}
+
 // Exclusive scan of one int per thread over the block, in thread order.
 __device__ __forceinline__ int block_excl_scan(int v, int &total) {
     __shared__ int sw[kWaves];

@@ -384,12 +384,12 @@ k_pair_mixed(Foam *foams, const int2 *bt, uint32_t flags) {
     mixed_block_dispatch<FLOW, false>(F, cx.lb, flags, soa, soar, near);
     const bool wave_near = __builtin_amdgcn_ballot_w64(near) != 0ull;
     const int wave_bit = 0 == (threadIdx.x & 63) && wave_near ? 1 << (threadIdx.x >> 6) : 0;
-    const int mark = block_sum_i(wave_bit);
-    soa = block_sum(soa);
-    soar = block_sum(soar);
+    double sums[2] = {soa, soar};
+    int mark = wave_bit;
+    block_reduce_t0(sums, 0u, mark);
     if (threadIdx.x == 0) {
-        F->s.part[cx.lb * kPartD + 2] = soa;
-        F->s.part[cx.lb * kPartD + 3] = soar;
+        F->s.part[cx.lb * kPartD + 2] = sums[0];
+        F->s.part[cx.lb * kPartD + 3] = sums[1];
         F->s.part_i[cx.lb * kPartI + 2] = mark;
     }
 }
@@ -409,11 +409,12 @@ __global__ __launch_bounds__(kBlock) void k_fix_mixed(Foam *foams, const int2 *b
     double soa = 0.0, soar = 0.0;
     bool near = false;
     mixed_block_dispatch<FLOW, true>(F, cx.lb, flags, soa, soar, near);
-    soa = block_sum(soa);
-    soar = block_sum(soar);
+    double sums[2] = {soa, soar};
+    int none = 0;
+    block_reduce_t0(sums, 0u, none);
     if (threadIdx.x == 0) {
-        F->s.part[cx.lb * kPartD + 2] = soa;
-        F->s.part[cx.lb * kPartD + 3] = soar;
+        F->s.part[cx.lb * kPartD + 2] = sums[0];
+        F->s.part[cx.lb * kPartD + 3] = sums[1];
     }
 }
 

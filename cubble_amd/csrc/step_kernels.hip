@@ -87,11 +87,12 @@ __global__ __launch_bounds__(kBlock) void k_predict(Foam *foams, const int2 *bt,
         if (gas) predict_dispatch<2, true>(c, R, p4, q4, base, ipt, n, h, sa, sar);
         else predict_dispatch<2, false>(c, R, p4, q4, base, ipt, n, h, sa, sar);
     }
-    sa = block_sum(sa);
-    sar = block_sum(sar);
+    double sums[2] = {sa, sar};
+    int none = 0;
+    block_reduce_t0(sums, 0u, none);
     if (threadIdx.x == 0) {
-        F->s.part[cx.lb * kPartD + 0] = sa;
-        F->s.part[cx.lb * kPartD + 1] = sar;
+        F->s.part[cx.lb * kPartD + 0] = sums[0];
+        F->s.part[cx.lb * kPartD + 1] = sums[1];
     }
 }
 
@@ -204,11 +205,12 @@ __global__ __launch_bounds__(kBlock) void k_pair(Foam *foams, const int2 *bt, ui
         if (gas) pair_range<2, true, FLOW>(c, R, S, cap, base, ipt, n, soa, soar);
         else pair_range<2, false, FLOW>(c, R, S, cap, base, ipt, n, soa, soar);
     }
-    soa = block_sum(soa);
-    soar = block_sum(soar);
+    double sums[2] = {soa, soar};
+    int none = 0;
+    block_reduce_t0(sums, 0u, none);
     if (threadIdx.x == 0) {
-        S.part[cx.lb * kPartD + 2] = soa;
-        S.part[cx.lb * kPartD + 3] = soar;
+        S.part[cx.lb * kPartD + 2] = sums[0];
+        S.part[cx.lb * kPartD + 3] = sums[1];
     }
 }
 
@@ -459,16 +461,15 @@ __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt,
     CorrAcc a{0.0, 0.0, 0.0, 0.0, 0};
     if (3 == c.dim) correct_dispatch<3, MIXED, WALLS>(c, R, S, dyn, base, ipt, n, flags, a);
     else correct_dispatch<2, MIXED, WALLS>(c, R, S, dyn, base, ipt, n, flags, a);
-    const double s_v = block_sum(a.s_v);
-    const double me = block_max(a.max_err);
-    const double mr = block_max(a.max_r);
-    const double mx = block_max(a.max_exp);
-    const int nd = block_sum_i(a.n_del);
+    // the volume sum, then the maxima of error, radius and expansion
+    double r4[4] = {a.s_v, a.max_err, a.max_r, a.max_exp};
+    int nd = a.n_del;
+    block_reduce_t0(r4, 0xeu, nd);
     if (threadIdx.x == 0) {
-        S.part[cx.lb * kPartD + 4] = s_v;
-        S.part[cx.lb * kPartD + 5] = me;
-        S.part[cx.lb * kPartD + 6] = mr;
-        S.part[cx.lb * kPartD + 7] = mx;
+        S.part[cx.lb * kPartD + 4] = r4[0];
+        S.part[cx.lb * kPartD + 5] = r4[1];
+        S.part[cx.lb * kPartD + 6] = r4[2];
+        S.part[cx.lb * kPartD + 7] = r4[3];
         S.part_i[cx.lb * kPartI + 0] = nd;
     }
 }
