@@ -171,16 +171,28 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
 }
 
 // Reduce the per-block partials [q0, q0 + nq) of a foam in fixed order.
-__device__ __forceinline__ double reduce_parts_sum(const double *part, int nb, int q) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) a += part[b * kPartD + q];
-    return block_sum(a);
-}
-
-__device__ __forceinline__ double reduce_parts_max(const double *part, int nb, int q) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) a = fmax(a, part[b * kPartD + q]);
-    return block_max(a);
+// Four consecutive slots q0..q0+3 of a foam's block partials, reduced in one
+// pass over the records (each thread takes blocks t, t + kBlock, ... in
+// order), and the integer slot qi of the int partials summed alongside when
+// qi >= 0.  Bit k of `dmax` makes slot q0 + k a maximum (from 0), else a
+// sum.  The results are defined in thread 0 only.
+__device__ __forceinline__ void reduce_parts4(const double *part, const int32_t *part_i, int nb,
+                                              int q0, unsigned dmax, int qi, double (&r)[4],
+                                              int &ri) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int ai = 0;
+#pragma unroll 2
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        const double *p = part + b * kPartD + q0;
+        const double v[4] = {p[0], p[1], p[2], p[3]};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = (dmax >> k) & 1u ? fmax(a[k], v[k]) : a[k] + v[k];
+        if (qi >= 0) ai += part_i[b * kPartI + qi];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = a[k];
+    ri = ai;
+    block_reduce_t0(r, dmax, ri);
 }
 
 // One kernel per FLOW value: the flow variant needs more registers, which
@@ -221,17 +233,14 @@ __global__ __launch_bounds__(kBlock) void k_pair(Foam *foams, const int2 *bt, ui
 __global__ __launch_bounds__(kBlock) void k_sums(Foam *foams) {
     Foam *F = foams + blockIdx.x;
     if (F->d.halt) return;
-    const double *part = F->s.part;
-    const int nb = F->nblk;
-    const double sa = reduce_parts_sum(part, nb, 0);
-    const double sar = reduce_parts_sum(part, nb, 1);
-    const double s_oa = reduce_parts_sum(part, nb, 2);
-    const double s_oar = reduce_parts_sum(part, nb, 3);
+    double r[4];
+    int none;
+    reduce_parts4(F->s.part, F->s.part_i, F->nblk, 0, 0u, -1, r, none);
     if (threadIdx.x == 0) {
-        F->d.s_a = sa;
-        F->d.s_ar = sar;
-        F->d.s_oa = s_oa;
-        F->d.s_oar = s_oar;
+        F->d.s_a = r[0];
+        F->d.s_ar = r[1];
+        F->d.s_oa = r[2];
+        F->d.s_oar = r[3];
     }
 }
 
@@ -479,17 +488,20 @@ __global__ __launch_bounds__(kBlock) void k_correct(Foam *foams, const int2 *bt,
 __global__ __launch_bounds__(kBlock) void k_control(Foam *foams, uint32_t flags) {
     Foam *F = foams + blockIdx.x;
     if (F->d.halt) return;
-    const Scratch S = F->s;
-    const int nb = F->nblk;
-    CorrAcc t;
-    t.s_v = reduce_parts_sum(S.part, nb, 4);
-    t.max_err = reduce_parts_max(S.part, nb, 5);
-    t.max_r = reduce_parts_max(S.part, nb, 6);
-    t.max_exp = reduce_parts_max(S.part, nb, 7);
-    int ndl = 0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) ndl += S.part_i[b * kPartI + 0];
-    t.n_del = block_sum_i(ndl);
-    if (threadIdx.x == 0) control(F, flags, t);
+    // the volume sum, the maxima of error, radius and expansion, the
+    // deletion count
+    double r[4];
+    int nd;
+    reduce_parts4(F->s.part, F->s.part_i, F->nblk, 4, 0xeu, 0, r, nd);
+    if (threadIdx.x == 0) {
+        CorrAcc t;
+        t.s_v = r[0];
+        t.max_err = r[1];
+        t.max_r = r[2];
+        t.max_exp = r[3];
+        t.n_del = nd;
+        control(F, flags, t);
+    }
 }
 
 // ---------------------------------------------------------------- launches
