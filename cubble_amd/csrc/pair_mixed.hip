@@ -157,16 +157,17 @@ struct MixedPair {
         // in the radii, so their order does not matter
         const float sq_i = ri_f * ri_f, sq_j = rj_f * rj_f;
         const float big = fmaxf(sq_i, sq_j);
-        // pairs the FP32 values do not settle
+        // pairs the FP32 values do not settle (each test evaluated: a wave
+        // rarely skips one, so no branches)
         bool special = m < close2;
         if (FLOW)
-            special = special || (j > i && fabsf(over) <= band_abs * rad + kBranchBand * (rad * rad));
+            special |= j > i && fabsf(over) <= band_abs * rad + kBranchBand * (rad * rad);
         if (GAS && D == 2) {
             const float t = fmaf(kTangentR, rad, tangent_s);
-            special = special || fabsf(over) <= t * t;
+            special |= fabsf(over) <= t * t;
         }
         // (the inside band uses max(r1, r2) <= (big + 1) / 2: one FMA)
-        if (GAS) special = special || fabsf(m - big) <= fmaf(big, inside_k1, inside_k0);
+        if (GAS) special |= fabsf(m - big) <= fmaf(big, inside_k1, inside_k0);
         bool overlap = over >= 0.f;
         bool inside = m < big;
         if (EXACT && special) {
@@ -193,8 +194,7 @@ struct MixedPair {
                     a2 = fminf(sq_i, sq_j);
                 } else {
                     const float dl = ri_f - rj_f;
-                    a2 = over * (m - dl * dl) * (0.25f * rs * rs);
-                    a2 = a2 < 0.f ? -a2 : a2;
+                    a2 = fabsf(over * (m - dl * dl) * (0.25f * rs * rs));
                 }
                 const float oa = 3 == D ? a2 * pi_f : 2.0f * __builtin_amdgcn_sqrtf(a2);
                 o.ovl += oa;
