@@ -24,8 +24,8 @@ struct Grid {
 
 void launch_predict(const Grid &g, uint32_t flags);
 void launch_pair(const Grid &g, uint32_t flags);
-// The mixed pair kernel, then (fix) the FP64 re-decision of the waves that
-// met a pair at a branch threshold.
+// The mixed pair kernel, then (fix) the second pass over the bubbles that met
+// a pair needing FP64 values.
 void launch_pair_mixed(const Grid &g, uint32_t flags, bool fix = true);
 // S_A, S_Ar, S_OA, S_OAr from the predictor and pair partials (one block per foam)
 void launch_sums(const Grid &g);

@@ -170,7 +170,6 @@ __device__ __forceinline__ void pair_range(const Consts &c, const Roles &R, cons
     }
 }
 
-// Reduce the per-block partials [q0, q0 + nq) of a foam in fixed order.
 // Four consecutive slots q0..q0+3 of a foam's block partials, reduced in one
 // pass over the records (each thread takes blocks t, t + kBlock, ... in
 // order), and the integer slot qi of the int partials summed alongside when
