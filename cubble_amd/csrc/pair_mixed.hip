@@ -44,11 +44,12 @@
 // every other term stays within 2^-15 R.  (In 3D the term is linear in the
 // difference.)
 //
-// A wave whose bubbles met such a pair marks itself, and a second kernel sums
-// the marked waves' bubbles again taking those pairs' d, |d|^2, R^2 - |d|^2
-// and both branch decisions from the FP64 predictions, exactly as the FP64
-// path evaluates them (rounded once to FP32 where the formula continues in
-// FP32), in parallel over the marked waves of every foam.
+// A bubble that met such a pair is flagged (the sign of its stored overlap
+// sum) and its workgroup marked; a second kernel sums each flagged bubble
+// again with one wave, taking those pairs' d, |d|^2, R^2 - |d|^2 and both
+// branch decisions from the FP64 predictions, exactly as the FP64 path
+// evaluates them (rounded once to FP32 where the formula continues in FP32),
+// in parallel over the marked workgroups of every foam.
 #include "dev_util.hpp"
 #include "launch.hpp"
 
