@@ -20,7 +20,8 @@ HIP graphs with autotuned launch shapes, K steps each.  The same line carries:
                          batching request loop: p50 / p99 latency, throughput;
   mixed_precision_error  the 3D 1e7-bubble foam: per-quantity error of a
                          mixed-precision attempt against FP64 from the same
-                         state, and the drift of a 500-step mixed run;
+                         state, and the drift of a 500-step mixed run; the
+                         same per-step error on the batch's largest 2D foam;
   cpu_baseline           the OpenMP FP64 restatement on the host cores, on a
                          bounded sample of the headline workload.
 Prints one JSON line.
@@ -376,27 +377,53 @@ def leg_mixed_error(ca, args, ip):
             "mean_radius_rel_diff": abs(float(np.mean(sb["x"][3])) / float(np.mean(sa["x"][3])) - 1),
             "gas_volume_rel_diff": abs(vol(sb["x"][3]) / vol(sa["x"][3]) - 1)}
     b.set_state(sa, sca)
-    a.step(ip)
-    b.step(ip)
-    ga, gb = a.get_attempt(), b.get_attempt()
-    f0 = sca["f0"]
-    s_r = max(float(np.max(np.abs(ga["vp"][3]))), 1e-300)
-    s_ov = max(float(np.max(np.abs(ga["ov"]))), 1e-300)
-    per_step = {
-        "velocity": max(scaled_err(gb["vp"][k], ga["vp"][k], f0) for k in range(3)),
-        "drdt": scaled_err(gb["vp"][3], ga["vp"][3], s_r),
-        "overlap_area": scaled_err(gb["ov"], ga["ov"], s_ov),
-        "position": max(scaled_err(gb["xc"][k], ga["xc"][k], 1.0) for k in range(3)),
-        "radius": scaled_err(gb["xc"][3], ga["xc"][3], 1.0),
-    }
+    per_step = attempt_error(a, b, 3, sca["f0"], ip)
     a.close()
     b.close()
-    log("mixed error: %s %s" % (json.dumps(per_step), json.dumps(traj)))
+    # the batch's largest 2D foam, where 2D overlap terms near tangency and
+    # nearly coincident centres of freshly seeded foams test the mixed path
+    c2 = max((c for c in ca.batch_workload(256, precision=ca.PREC_FP64) if c.dim == 2),
+             key=lambda c: c.num_start)
+    per_step_2d = {}
+    for steps in (3, 50):
+        a = ca.Simulator(c2)
+        b = ca.Simulator(dataclasses.replace(c2, precision=ca.PREC_MIXED))
+        a.run_steps(steps, ip)
+        sa2, sc2 = a.get_state()
+        b.set_state(sa2, sc2)
+        per_step_2d["after_%d_steps" % steps] = attempt_error(a, b, 2, sc2["f0"], ip)
+        a.close()
+        b.close()
+    log("mixed error: %s %s 2D %s" % (json.dumps(per_step), json.dumps(traj),
+                                      json.dumps(per_step_2d)))
     return {"workload": "3D periodic foam, 10,000,000 bubbles, log-normal radii sigma 0.15, "
                         "phi 0.9", "steps": args.mixed_steps,
             "per_step_vs_fp64": per_step, "metric": "max |mixed - fp64| / max(|fp64|, scale); "
             "scale = f0 for velocities, max|q| for dR/dt and overlap, 1 for positions and radii",
-            "run_vs_fp64": traj}
+            "run_vs_fp64": traj,
+            "per_step_vs_fp64_2d": {
+                "workload": "the batch's largest 2D foam: %d bubbles, normal radii, phi %.3f, "
+                            "FP64 state after 3 and after 50 steps" % (c2.num_start, c2.phi),
+                **per_step_2d}}
+
+
+def attempt_error(a, b, dim, f0, ip):
+    """One attempt of FP64 simulator `a` and mixed-precision simulator `b`
+    from the same state, compared quantity by quantity (scaled metric)."""
+    import numpy as np
+
+    a.step(ip)
+    b.step(ip)
+    ga, gb = a.get_attempt(), b.get_attempt()
+    s_r = max(float(np.max(np.abs(ga["vp"][3]))), 1e-300)
+    s_ov = max(float(np.max(np.abs(ga["ov"]))), 1e-300)
+    return {
+        "velocity": max(scaled_err(gb["vp"][k], ga["vp"][k], f0) for k in range(dim)),
+        "drdt": scaled_err(gb["vp"][3], ga["vp"][3], s_r),
+        "overlap_area": scaled_err(gb["ov"], ga["ov"], s_ov),
+        "position": max(scaled_err(gb["xc"][k], ga["xc"][k], 1.0) for k in range(dim)),
+        "radius": scaled_err(gb["xc"][3], ga["xc"][3], 1.0),
+    }
 
 
 def leg_serve(ca, args, ip):
