@@ -41,6 +41,9 @@ def _inside_count(sim):
     (3, 5000, 43, ca.PREC_FP64, (1, 0, 0)),
     (2, 5000, 44, ca.PREC_MIXED, (0, 0, 0)),
     (3, 8000, 45, ca.PREC_MIXED, (0, 1, 0)),
+    # a large 2D box with walls, a few steps after seeding: the flow's own
+    # branch band next to nearly coincident and near-tangent pairs
+    (2, 300000, 46, ca.PREC_MIXED, (1, 1, 0)),
 ])
 def test_flow_attempt(ora, dim, n, seed, precision, wall):
     print(f"flow attempt dim={dim} n={n} seed={seed} precision={precision} wall={wall}",
