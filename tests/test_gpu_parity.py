@@ -222,16 +222,18 @@ def test_walls_first_search_and_attempt_fp64(ora, dim, n, seed, wall):
     sim.close()
 
 
-# the last two: 2D boxes of ~1,030 and ~1,460 radii with walls (wall axes code
-# positions in units of L / 2^30), three steps after seeding: they hold nearly
-# coincident centres (where code quantisation turns the pair direction) and
-# many near-tangent pairs (where it moves the 2D overlap term), whose values
-# the mixed kernel takes from FP64
+# the 2D cases of 3e5 and 6e5 bubbles: boxes of ~1,030 and ~1,460 radii with
+# walls (wall axes code positions in units of L / 2^30), three steps after
+# seeding: they hold nearly coincident centres (where code quantisation turns
+# the pair direction) and many near-tangent pairs (where it moves the 2D
+# overlap term), whose values the mixed kernel takes from FP64; the last, a
+# 3D box of 4e5 bubbles with walls on every axis
 @pytest.mark.parametrize("dim,n,seed,wall", [(2, 5000, 31, (0, 0, 0)), (3, 8000, 32, (0, 0, 0)),
                                              (3, 60000, 33, (0, 0, 0)), (2, 4000, 34, (0, 1, 0)),
                                              (3, 6000, 35, (1, 0, 1)),
                                              (2, 300000, 36, (1, 1, 0)),
-                                             (2, 600000, 37, (1, 0, 0))])
+                                             (2, 600000, 37, (1, 0, 0)),
+                                             (3, 400000, 38, (1, 1, 1))])
 def test_mixed_precision_attempt_error(ora, dim, n, seed, wall):
     cfg = config(dim, n, seed=seed, precision=ca.PREC_MIXED, wall=wall)
     sim = ca.Simulator(cfg)
