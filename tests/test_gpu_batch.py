@@ -50,6 +50,27 @@ def test_batch_run_steps_equals_alone(precision):
     batch.close()
 
 
+@pytest.mark.parametrize("precision", [ca.PREC_FP64, ca.PREC_MIXED])
+def test_batch_with_wall_foams_equals_alone(precision):
+    """Foams with walls and periodic ones in one batch: the launch takes the
+    corrector variant with the wall code for all of them, and every foam
+    still equals its run alone (where a periodic foam takes the variant
+    without it), bit for bit."""
+    print(f"batch with walls precision={precision}", flush=True)
+    specs = [(2, 3000, (0, 0, 0)), (2, 2500, (1, 1, 0)), (3, 4000, (1, 0, 1)),
+             (3, 3500, (0, 0, 0))]
+    cfgs = [config(d, n, seed=200 + k, precision=precision, wall=w)
+            for k, (d, n, w) in enumerate(specs)]
+    batch = ca.Batch(cfgs)
+    batch.run_steps(30, GAS)
+    for k, cfg in enumerate(cfgs):
+        sim = ca.Simulator(cfg)
+        sim.run_steps(30, GAS)
+        assert_same(batch.get_state(k), sim.get_state())
+        sim.close()
+    batch.close()
+
+
 def test_batch_integrate_equals_alone():
     cfgs = mixed_configs()[:4]
     batch = ca.Batch(cfgs)
