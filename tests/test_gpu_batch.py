@@ -71,6 +71,27 @@ def test_batch_with_wall_foams_equals_alone(precision):
     batch.close()
 
 
+@pytest.mark.parametrize("precision", [ca.PREC_FP64, ca.PREC_MIXED])
+def test_batch_with_flow_equals_alone(precision):
+    """The imposed flow in a batch (each foam its own flow box and velocity,
+    one with walls): every foam equals its run alone, bit for bit."""
+    print(f"batch with flow precision={precision}", flush=True)
+    flow = ca.IntegrationParams(True, True, True)
+    specs = [(2, 3000, (0.5, -0.25, 0.0), (0, 0, 0)), (3, 4000, (0.2, 0.1, -0.3), (0, 0, 0)),
+             (2, 2500, (-0.4, 0.3, 0.0), (1, 0, 0))]
+    cfgs = [config(d, n, seed=300 + k, precision=precision, wall=w, flow=1, flow_vel=v,
+                   flow_lbb=(0.3, 0.35, 0.3), flow_tfr=(0.7, 0.6, 0.65))
+            for k, (d, n, v, w) in enumerate(specs)]
+    batch = ca.Batch(cfgs)
+    batch.run_steps(30, flow)
+    for k, cfg in enumerate(cfgs):
+        sim = ca.Simulator(cfg)
+        sim.run_steps(30, flow)
+        assert_same(batch.get_state(k), sim.get_state())
+        sim.close()
+    batch.close()
+
+
 def test_batch_integrate_equals_alone():
     cfgs = mixed_configs()[:4]
     batch = ca.Batch(cfgs)
