@@ -310,6 +310,42 @@ def test_mixed_close_and_tangent_pairs(ora, dim, wall):
     sim.close()
 
 
+def test_mixed_second_pass_with_several_bubbles_per_thread(ora):
+    """The mixed second pass gathers each bubble slot's flagged bubbles in
+    turn when a thread holds several bubbles: with 1, 2 and 4 bubbles per
+    thread the per-bubble terms (velocities, overlaps) are the same bits,
+    dR/dt differs only through the block partials' summation order, and each
+    run meets the oracle bar.  A large 2D box with walls, three steps after
+    seeding, flags many bubbles."""
+    cfg = config(2, 300000, seed=36, precision=ca.PREC_MIXED, wall=(1, 1, 0))
+    base = ca.Simulator(cfg)
+    for _ in range(3):
+        base.integrate(GAS)
+    st, sc = base.get_state()
+    base.close()
+    outs = []
+    for ipt in (1, 2, 4):
+        print(f"mixed second pass bubbles_per_thread={ipt}", flush=True)
+        sim = ca.Simulator(cfg, tuning={"bubbles_per_thread": ipt, "search_tile": 64,
+                                        "graph_steps": 8})
+        sim.set_state(st, sc)
+        f = oracle_from_sim(ora, sim)
+        sim.step(GAS)
+        f.attempt(True, True)
+        g = sim.get_attempt()
+        errs = attempt_errors(2, g, f.attempt_out(), sc["f0"])
+        for q, e in errs.items():
+            assert e <= MIXED_TOL, (ipt, q, e)
+        outs.append(g)
+        sim.close()
+    for g in outs[1:]:
+        for k in range(2):
+            assert np.array_equal(g["vp"][k], outs[0]["vp"][k])
+        assert np.array_equal(g["ov"], outs[0]["ov"])
+        assert scaled_err(g["vp"][3], outs[0]["vp"][3],
+                          float(np.max(np.abs(outs[0]["vp"][3])))) <= FP64_TOL
+
+
 def test_large_foam_multiblock_partials(ora):
     """More than 256 * 4096 bubbles: several bubbles per thread, the full
     reduction tree and the XCD block remap."""
